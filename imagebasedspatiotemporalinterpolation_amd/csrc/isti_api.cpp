@@ -1,0 +1,383 @@
+// isti_api.cpp - the C ABI of include/isti.h: renderer state, device scratch
+// and the per-frame stage sequence of Renderer::render
+// (src/renderer.cu:549-694).
+//
+// Per frame (modes 0-3) the host issues three stream-ordered launches and
+// nothing else: splat clear, one reproject+splat launch covering every
+// source view, one resolve launch. No host synchronisation and no
+// host->device copies happen inside isti_render, so a frame can be captured
+// into a hipGraph. Scratch is (re)allocated only when a view's size changes.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "../../include/isti.h"
+#include "isti_kernels.hpp"
+
+using namespace isti;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct Scratch {
+    uint32_t w = 0, h = 0;
+    unsigned long long *splat = nullptr;
+    float4 *cam_ff = nullptr;
+    float4 *full_ff = nullptr;
+};
+
+}  // namespace
+
+struct isti_renderer {
+    int device = 0;
+    int n_aux = 0;
+    isti_view_desc views[2][MAX_VIEWS];
+    bool view_set[2][MAX_VIEWS];
+    Scratch scratch[2][MAX_VIEWS];
+    double *row_sums = nullptr;
+    uint32_t row_sums_cap = 0;
+    float *mean_z = nullptr;
+    std::string err;
+};
+
+namespace {
+
+isti_status fail(isti_renderer *r, isti_status st, const std::string &msg) {
+    if (r) r->err = msg;
+    return st;
+}
+
+isti_status dev_fail(isti_renderer *r, hipError_t e, const char *where) {
+    return fail(r, e == hipErrorOutOfMemory ? ISTI_ERR_OOM : ISTI_ERR_DEVICE,
+                std::string(where) + ": " + hipGetErrorString(e));
+}
+
+void free_scratch(Scratch &s) {
+    if (s.splat) (void)hipFree(s.splat);
+    if (s.cam_ff) (void)hipFree(s.cam_ff);
+    if (s.full_ff) (void)hipFree(s.full_ff);
+    s = Scratch{};
+}
+
+isti_status ensure_scratch(isti_renderer *r, Scratch &s, uint32_t w, uint32_t h) {
+    if (s.w == w && s.h == h && s.splat) return ISTI_OK;
+    free_scratch(s);
+    const size_t n = (size_t)w * h;
+    hipError_t e = hipMalloc(&s.splat, n * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&s.cam_ff, n * sizeof(float4));
+    if (e == hipSuccess) e = hipMalloc(&s.full_ff, n * sizeof(float4));
+    if (e != hipSuccess) {
+        free_scratch(s);
+        return dev_fail(r, e, "hipMalloc(scratch)");
+    }
+    s.w = w;
+    s.h = h;
+    return ISTI_OK;
+}
+
+Mat4 to_mat(const float *m) {
+    Mat4 M;
+    std::memcpy(M.m, m, sizeof(M.m));
+    return M;
+}
+
+bool has_motion(const isti_view_desc &d) { return d.motion_x && d.motion_y && d.motion_z; }
+
+ResolveView resolve_view(const isti_view_desc &d, const Scratch &s, bool forward, int priority) {
+    ResolveView v;
+    v.img = reinterpret_cast<const uchar4 *>(d.color);
+    v.z = d.z;
+    v.sh_info = reinterpret_cast<const uchar4 *>(d.shadow_info);
+    v.sh_a = reinterpret_cast<const uchar4 *>(d.shadow_color_a);
+    v.sh_b = reinterpret_cast<const uchar4 *>(d.shadow_color_b);
+    v.cam_ff = s.cam_ff;
+    v.full_ff = s.full_ff;
+    v.splat = s.splat;
+    v.w = d.width;
+    v.h = d.height;
+    v.priority = priority;
+    v.forward = forward ? 1 : 0;
+    return v;
+}
+
+isti_status validate(isti_renderer *r, const isti_render_info *info, size_t out_pitch) {
+    if (info->mode < ISTI_MODE_OURS || info->mode > ISTI_MODE_TIMEWARP)
+        return fail(r, ISTI_ERR_INVALID_ARG, "mode must be 0..4");
+    if (info->width == 0 || info->height == 0 || info->width > ISTI_MAX_DIM || info->height > ISTI_MAX_DIM)
+        return fail(r, ISTI_ERR_UNSUPPORTED_SHAPE, "render resolution must be 1..65535 per axis");
+    if (out_pitch < (size_t)info->width * 4 || out_pitch % 4 != 0)
+        return fail(r, ISTI_ERR_INVALID_ARG, "out_pitch_bytes must be a multiple of 4 and >= width*4");
+    const int nv = info->mode == ISTI_MODE_TIMEWARP ? 1 : 1 + r->n_aux;
+    for (int s = 0; s < (info->mode == ISTI_MODE_TIMEWARP ? 1 : 2); s++)
+        for (int v = 0; v < nv; v++)
+            if (!r->view_set[s][v]) return fail(r, ISTI_ERR_INVALID_ARG, "keyframe view not set");
+    if (info->mode == ISTI_MODE_TIMEWARP) return ISTI_OK;
+    const isti_view_desc &f = r->views[0][0], &b = r->views[1][0];
+    // renderer.cu:145,153: splat buffers are indexed by keyframe dims but
+    // sized by the render resolution, so the two must agree.
+    if (f.width != info->width || f.height != info->height || b.width != info->width || b.height != info->height)
+        return fail(r, ISTI_ERR_UNSUPPORTED_SHAPE, "main keyframes must have the render resolution");
+    if (info->shadow_frames_per_anchor < 0 || info->shadow_frames_per_anchor > ISTI_MAX_SHADOW_FRAMES)
+        return fail(r, ISTI_ERR_INVALID_ARG, "shadow_frames_per_anchor must be 0..12");
+    for (int v = 1; v <= r->n_aux; v++) {
+        const isti_view_desc &af = r->views[0][v], &ab = r->views[1][v];
+        if (af.width != ab.width || af.height != ab.height)
+            return fail(r, ISTI_ERR_UNSUPPORTED_SHAPE, "aux keyframes of one camera must share a resolution");
+        // renderer.cu:433: aux pixel = render pixel / (render res / aux res)
+        if (af.width > info->width || af.height > info->height || info->width % af.width != 0 ||
+            info->height % af.height != 0)
+            return fail(r, ISTI_ERR_UNSUPPORTED_SHAPE, "render resolution must be an integer multiple of each aux resolution");
+    }
+    return ISTI_OK;
+}
+
+isti_status render_timewarp(isti_renderer *r, const isti_render_info *info, uint8_t *out, size_t pitch,
+                            hipStream_t s) {
+    // src/renderer.cu:615-629
+    const isti_view_desc &f = r->views[0][0];
+    if (r->row_sums_cap < f.height) {
+        if (r->row_sums) (void)hipFree(r->row_sums);
+        r->row_sums = nullptr;
+        r->row_sums_cap = 0;
+        hipError_t e = hipMalloc(&r->row_sums, sizeof(double) * f.height);
+        if (e != hipSuccess) return dev_fail(r, e, "hipMalloc(row sums)");
+        r->row_sums_cap = f.height;
+    }
+    if (!r->mean_z) {
+        hipError_t e = hipMalloc(&r->mean_z, sizeof(float));
+        if (e != hipSuccess) return dev_fail(r, e, "hipMalloc(mean z)");
+    }
+    hipError_t e = launch_mean_z(f.z, f.width, f.height, r->row_sums, r->mean_z, s);
+    if (e != hipSuccess) return dev_fail(r, e, "mean z launch");
+
+    TimewarpArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.inv_vp_from = to_mat(info->inv_viewproj);
+    a.inv_vp_to = to_mat(f.inv_viewproj);
+    a.vp_to = to_mat(f.viewproj);
+    a.depth_to = r->mean_z;
+    a.img_to = reinterpret_cast<const uchar4 *>(f.color);
+    a.w_to = f.width;
+    a.h_to = f.height;
+    if (r->n_aux > 0 && r->view_set[0][1]) {
+        const isti_view_desc &x = r->views[0][1];
+        a.aux_set = 1;
+        a.vp_aux = to_mat(x.viewproj);
+        a.img_aux = reinterpret_cast<const uchar4 *>(x.color);
+        a.w_aux = x.width;
+        a.h_aux = x.height;
+    }
+    a.out = reinterpret_cast<uchar4 *>(out);
+    a.out_pitch_px = (uint32_t)(pitch / 4);
+    a.w = info->width;
+    a.h = info->height;
+    a.tiles_x = (a.w + TILE_W - 1) / TILE_W;
+    a.n_tiles = a.tiles_x * ((a.h + TILE_H - 1) / TILE_H);
+    e = launch_timewarp(a, s);
+    if (e != hipSuccess) return dev_fail(r, e, "timewarp launch");
+    return ISTI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+isti_status isti_create(const isti_config *config, isti_renderer **out) {
+    if (!config || !out) {
+        g_create_error = "config and out must be non-null";
+        return ISTI_ERR_INVALID_ARG;
+    }
+    *out = nullptr;
+    if (config->n_aux < 0 || config->n_aux > ISTI_MAX_AUX) {
+        g_create_error = "n_aux must be 0..4";
+        return ISTI_ERR_INVALID_ARG;
+    }
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) {
+        g_create_error = std::string("no HIP device: ") + hipGetErrorString(e);
+        return ISTI_ERR_DEVICE;
+    }
+    if (config->device < 0 || config->device >= count) {
+        g_create_error = "device ordinal out of range";
+        return ISTI_ERR_INVALID_ARG;
+    }
+    isti_renderer *r = new (std::nothrow) isti_renderer();
+    if (!r) {
+        g_create_error = "out of host memory";
+        return ISTI_ERR_OOM;
+    }
+    r->device = config->device;
+    r->n_aux = config->n_aux;
+    std::memset(r->views, 0, sizeof(r->views));
+    std::memset(r->view_set, 0, sizeof(r->view_set));
+    *out = r;
+    return ISTI_OK;
+}
+
+isti_status isti_destroy(isti_renderer *r) {
+    if (!r) return ISTI_ERR_INVALID_ARG;
+    (void)hipSetDevice(r->device);
+    for (auto &slot : r->scratch)
+        for (auto &s : slot) free_scratch(s);
+    if (r->row_sums) (void)hipFree(r->row_sums);
+    if (r->mean_z) (void)hipFree(r->mean_z);
+    delete r;
+    return ISTI_OK;
+}
+
+isti_status isti_keyframe_set(isti_renderer *r, int32_t slot, int32_t view, const isti_view_desc *d) {
+    if (!r) return ISTI_ERR_INVALID_ARG;
+    if (!d || slot < 0 || slot > 1 || view < 0 || view > r->n_aux)
+        return fail(r, ISTI_ERR_INVALID_ARG, "slot must be 0/1 and view 0..n_aux, desc non-null");
+    if (d->width == 0 || d->height == 0 || d->width > ISTI_MAX_DIM || d->height > ISTI_MAX_DIM)
+        return fail(r, ISTI_ERR_UNSUPPORTED_SHAPE, "view size must be 1..65535 per axis");
+    if (!d->color || !d->z) return fail(r, ISTI_ERR_INVALID_ARG, "color and z planes are required");
+    const int nm = (d->motion_x != nullptr) + (d->motion_y != nullptr) + (d->motion_z != nullptr);
+    if (nm != 0 && nm != 3) return fail(r, ISTI_ERR_INVALID_ARG, "motion planes: all three or none");
+    r->views[slot][view] = *d;
+    r->view_set[slot][view] = true;
+    return ISTI_OK;
+}
+
+isti_status isti_keyframe_swap(isti_renderer *r) {
+    if (!r) return ISTI_ERR_INVALID_ARG;
+    for (int v = 0; v < MAX_VIEWS; v++) {
+        std::swap(r->views[0][v], r->views[1][v]);
+        std::swap(r->view_set[0][v], r->view_set[1][v]);
+    }
+    return ISTI_OK;
+}
+
+isti_status isti_render(isti_renderer *r, const isti_render_info *info, uint8_t *out, size_t out_pitch,
+                        void *stream) {
+    if (!r) return ISTI_ERR_INVALID_ARG;
+    if (!info || !out) return fail(r, ISTI_ERR_INVALID_ARG, "info and out must be non-null");
+    isti_status st = validate(r, info, out_pitch);
+    if (st != ISTI_OK) return st;
+    hipError_t e = hipSetDevice(r->device);
+    if (e != hipSuccess) return dev_fail(r, e, "hipSetDevice");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+
+    if (info->mode == ISTI_MODE_TIMEWARP) return render_timewarp(r, info, out, out_pitch, s);
+
+    const int nv = 1 + r->n_aux;
+    for (int sl = 0; sl < 2; sl++)
+        for (int v = 0; v < nv; v++) {
+            st = ensure_scratch(r, r->scratch[sl][v], r->views[sl][v].width, r->views[sl][v].height);
+            if (st != ISTI_OK) return st;
+        }
+
+    // splat clear + reproject/splat: jobs ordered fwd main, bwd main, then
+    // per aux camera curr / next, with the reference's weights and matrices
+    // (src/renderer.cu:643-649)
+    ClearArgs ca;
+    std::memset(&ca, 0, sizeof(ca));
+    SplatArgs sa;
+    std::memset(&sa, 0, sizeof(sa));
+    sa.vp_cam = to_mat(info->viewproj);
+    sa.vp_main[0] = to_mat(r->views[0][0].viewproj);
+    sa.vp_main[1] = to_mat(r->views[1][0].viewproj);
+    sa.init_linear = info->mode == ISTI_MODE_BACKWARD_BIDIRECTIONAL ? 1 : 0;
+    uint32_t clear_blocks = 0, splat_blocks = 0;
+    int nj = 0;
+    for (int v = 0; v < nv; v++) {
+        for (int sl = 0; sl < 2; sl++) {
+            const isti_view_desc &d = r->views[sl][v];
+            const isti_view_desc &other = r->views[1 - sl][v];
+            Scratch &sc = r->scratch[sl][v];
+            const uint32_t n = d.width * d.height;
+            ca.buf[nj] = sc.splat;
+            ca.n[nj] = n;
+            ca.block_begin[nj] = clear_blocks;
+            clear_blocks += (n + 511u) / 512u;
+
+            SplatJob &J = sa.job[nj];
+            const bool hm = has_motion(d);
+            J.z = d.z;
+            J.mx = hm ? d.motion_x : nullptr;
+            J.my = hm ? d.motion_y : nullptr;
+            J.mz = hm ? d.motion_z : nullptr;
+            J.full_ff = sc.full_ff;
+            J.cam_ff = sc.cam_ff;
+            J.splat = sc.splat;
+            J.w = d.width;
+            J.h = d.height;
+            J.n_px = n;
+            J.block_begin = splat_blocks;
+            splat_blocks += (n + SPLAT_BLOCK - 1) / SPLAT_BLOCK;
+            J.weight = sl == 0 ? info->alpha : 1.0f - info->alpha;
+            J.main_slot = sl;
+            J.inv_vp_from = to_mat(d.inv_viewproj);
+            J.vp_to = to_mat(other.viewproj);
+            nj++;
+        }
+    }
+    ca.n_bufs = nj;
+    ca.n_blocks = clear_blocks;
+    sa.n_jobs = nj;
+    sa.n_blocks = splat_blocks;
+
+    ResolveArgs ra;
+    std::memset(&ra, 0, sizeof(ra));
+    for (int v = 0; v < nv; v++) {
+        const int prio_f = v == 0 ? 0 : r->views[0][v].priority;  // main priority is 0 (renderer.cu:654)
+        const int prio_b = v == 0 ? 0 : r->views[1][v].priority;
+        ra.fwd[v] = resolve_view(r->views[0][v], r->scratch[0][v], true, prio_f);
+        ra.bwd[v] = resolve_view(r->views[1][v], r->scratch[1][v], false, prio_b);
+    }
+    ra.out = reinterpret_cast<uchar4 *>(out);
+    ra.out_pitch_px = (uint32_t)(out_pitch / 4);
+    ra.w = info->width;
+    ra.h = info->height;
+    ra.alpha = info->alpha;
+    ra.fpa = info->frames_per_anchor;
+    ra.sfpa = info->shadow_frames_per_anchor;
+    ra.no_shadows = info->no_shadows;
+    ra.n_aux = r->n_aux;
+    int window = 1, iters = 3;
+    bool always_valid = false;
+    switch (info->mode) {  // src/renderer.cu:676-693
+        case ISTI_MODE_OURS:
+        case ISTI_MODE_BACKWARD_UNIDIRECTIONAL: window = 1; iters = 3; break;
+        case ISTI_MODE_SPLATTING: window = 1; iters = 0; break;
+        case ISTI_MODE_BACKWARD_BIDIRECTIONAL: window = 3; iters = 5; always_valid = true; break;
+        default: break;
+    }
+    ra.max_iter = info->max_search_iterations >= 0 ? info->max_search_iterations : iters;
+    ra.tiles_x = (ra.w + TILE_W - 1) / TILE_W;
+    ra.n_tiles = ra.tiles_x * ((ra.h + TILE_H - 1) / TILE_H);
+
+    e = launch_clear(ca, s);
+    if (e != hipSuccess) return dev_fail(r, e, "clear launch");
+    e = launch_splat(sa, s);
+    if (e != hipSuccess) return dev_fail(r, e, "splat launch");
+    e = launch_resolve(ra, window, always_valid, s);
+    if (e != hipSuccess) return dev_fail(r, e, "resolve launch");
+    return ISTI_OK;
+}
+
+isti_status isti_scratch(const isti_renderer *r, int32_t slot, int32_t view, isti_scratch_view *out) {
+    if (!r || !out || slot < 0 || slot > 1 || view < 0 || view > r->n_aux) return ISTI_ERR_INVALID_ARG;
+    const Scratch &s = r->scratch[slot][view];
+    out->width = s.w;
+    out->height = s.h;
+    out->splat = reinterpret_cast<const uint64_t *>(s.splat);
+    out->cam_ff = reinterpret_cast<const float *>(s.cam_ff);
+    out->full_ff = reinterpret_cast<const float *>(s.full_ff);
+    return ISTI_OK;
+}
+
+const char *isti_last_error(const isti_renderer *r) {
+    if (!r) return g_create_error.c_str();
+    return r->err.c_str();
+}
+
+const char *isti_version(void) { return "isti-gfx950 0.1.0"; }
+
+}  // extern "C"

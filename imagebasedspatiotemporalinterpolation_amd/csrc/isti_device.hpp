@@ -1,0 +1,122 @@
+// isti_device.hpp - gfx950 device math for the interpolation path.
+//
+// Every helper states the reference lines whose semantics it implements.
+// Numerics contract (shared with the CPU oracle, checked bit-for-bit by the
+// parity tests): compiled with -ffp-contract=off and IEEE-correct fp32
+// division and square root, one binary32 operation per operator in the
+// order the reference writes it; 4x4 matrix-vector products associate as
+// (m0*v.x + m1*v.y) + (m2*v.z + m3*v.w) (glm's mat4*vec4 order).
+// float->int conversions follow the GPU semantics the reference ran with:
+// truncation toward zero, NaN -> 0, saturation at the integer range.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cfloat>
+
+namespace isti {
+
+struct Mat4 {
+    float m[16];  // column-major, (row r, col c) at m[c*4+r]
+};
+
+// src/renderer.cu:16-22, 27-28
+constexpr float EPS_1_FULL_RES = 0.002f;
+constexpr float EPS_1_LOW_RES = EPS_1_FULL_RES * 4.0f;
+constexpr float EPS_2_FULL_RES = 0.001f;
+constexpr float EPS_2_LOW_RES = 0.004f;
+constexpr float EPS_3 = 0.01f;
+constexpr unsigned long long SPLAT_EMPTY = 0xfefefefefefefefeull;  // memset(0xfe), renderer.cu:22
+constexpr uint32_t SOFTSHADOW_BITS = 2;
+constexpr uint32_t SOFTSHADOW_VAL_MASK = (1u << SOFTSHADOW_BITS) - 1u;
+
+__device__ __forceinline__ int32_t cvt_s32(float f) {
+    if (f != f) return 0;
+    if (f >= 2147483648.0f) return INT32_MAX;
+    if (f <= -2147483648.0f) return INT32_MIN;
+    return (int32_t)f;
+}
+// (unsigned char)(float) of make_uchar4(float4), helper_math_ext.h:19-22
+__device__ __forceinline__ uint8_t cvt_u8(float f) {
+    if (f != f) return 0;
+    if (f <= 0.0f) return 0;
+    if (f >= 255.0f) return 255;
+    return (uint8_t)f;
+}
+__device__ __forceinline__ uint32_t sat_u32(float r) {
+    if (r != r) return 0;
+    if (r <= 0.0f) return 0;
+    if (r >= 4294967296.0f) return 0xffffffffu;
+    return (uint32_t)r;
+}
+// __float2uint_rn / _rd / _ru, renderer.cu:359-362
+__device__ __forceinline__ uint32_t f2u_rn(float f) { return sat_u32(__builtin_rintf(f)); }
+__device__ __forceinline__ uint32_t f2u_rd(float f) { return sat_u32(floorf(f)); }
+__device__ __forceinline__ uint32_t f2u_ru(float f) { return sat_u32(ceilf(f)); }
+
+// helper_common.h:8-11
+__device__ __forceinline__ bool in_range(int2 p, uint32_t w, uint32_t h) {
+    return p.x >= 0 && (uint32_t)p.x < w && p.y >= 0 && (uint32_t)p.y < h;
+}
+// helper_common.h:66-69
+__device__ __forceinline__ float2 pixel2cs(uint32_t x, uint32_t y, uint32_t w, uint32_t h) {
+    float2 r;
+    r.x = 2.0f * ((float)x + 0.5f) / (float)w - 1.0f;
+    r.y = 2.0f * ((float)y + 0.5f) / (float)h - 1.0f;
+    return r;
+}
+// helper_common.h:71-75
+__device__ __forceinline__ int2 cs2pixel(float cx, float cy, uint32_t w, uint32_t h) {
+    int2 r;
+    r.x = cvt_s32(floorf((cx + 1.0f) * (float)w * 0.5f));
+    r.y = cvt_s32(floorf((cy + 1.0f) * (float)h * 0.5f));
+    return r;
+}
+__device__ __forceinline__ float4 mat_vec(const Mat4 &M, float4 v) {
+    const float *m = M.m;
+    float4 r;
+    r.x = (m[0] * v.x + m[4] * v.y) + (m[8] * v.z + m[12] * v.w);
+    r.y = (m[1] * v.x + m[5] * v.y) + (m[9] * v.z + m[13] * v.w);
+    r.z = (m[2] * v.x + m[6] * v.y) + (m[10] * v.z + m[14] * v.w);
+    r.w = (m[3] * v.x + m[7] * v.y) + (m[11] * v.z + m[15] * v.w);
+    return r;
+}
+// helper_common.h:77-87 CS2WS / WS2CS
+__device__ __forceinline__ float4 project(const Mat4 &M, float4 v) {
+    const float4 r = mat_vec(M, v);
+    const float w = r.w;
+    return make_float4(r.x / w, r.y / w, r.z / w, r.w / w);
+}
+// helper_common.h:89-95
+__device__ __forceinline__ float z_to_depth(float z) { return (z + 1.0f) * 0.5f; }
+__device__ __forceinline__ float depth_to_z(float d) { return d * 2.0f - 1.0f; }
+
+// helper_common.h:22-38: lerp_color then color_to_uchar4 (to_8bit truncates)
+__device__ __forceinline__ uint8_t lerp_ch(uint32_t ua, uint32_t ub, float t) {
+    const float fa = (float)ua, fb = (float)ub;
+    const float l = (fa + t * (fb - fa)) / 255.f;
+    const int32_t v = cvt_s32(l * 255.f);
+    return (uint8_t)(v < 0 ? 0 : (v > 255 ? 255 : v));
+}
+__device__ __forceinline__ uchar4 lerp_to_u8(uchar4 a, uchar4 b, float t) {
+    return make_uchar4(lerp_ch(a.x, b.x, t), lerp_ch(a.y, b.y, t), lerp_ch(a.z, b.z, t), lerp_ch(a.w, b.w, t));
+}
+
+// renderer.cu:59-74 splat key: depth bits << 32 | x << 16 | y
+__device__ __forceinline__ unsigned long long encode_key(float depth, uint32_t x, uint32_t y) {
+    return ((unsigned long long)__float_as_uint(depth) << 32) | (unsigned long long)((x << 16) | y);
+}
+__device__ __forceinline__ float key_depth(unsigned long long k) { return __uint_as_float((uint32_t)(k >> 32)); }
+__device__ __forceinline__ uint32_t key_x(unsigned long long k) { return (uint32_t)k >> 16; }
+__device__ __forceinline__ uint32_t key_y(unsigned long long k) { return (uint32_t)(k & 0xFFFFull); }
+
+// Point sample with border addressing (src/util/image_buffer.cu:227-249):
+// out of range, or a missing plane, reads as zero.
+template <typename T>
+__device__ __forceinline__ T fetch(const T *plane, uint32_t w, uint32_t h, uint32_t x, uint32_t y) {
+    if (plane == nullptr || x >= w || y >= h) return T{};
+    return plane[(size_t)y * w + x];
+}
+
+}  // namespace isti

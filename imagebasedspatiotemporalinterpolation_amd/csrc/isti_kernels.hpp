@@ -1,0 +1,102 @@
+// isti_kernels.hpp - kernel argument blocks and launchers of the gfx950
+// interpolation path. Every per-frame descriptor travels as a by-value
+// kernel argument, so a frame needs no host->device copy and no host sync
+// (the reference copies aux descriptors with cudaMemcpy and syncs the host
+// between its two kernels, src/renderer.cu:650,670-671).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/isti.h"
+#include "isti_device.hpp"
+
+namespace isti {
+
+constexpr int MAX_VIEWS = 1 + ISTI_MAX_AUX;        // main + aux per slot
+constexpr int MAX_SPLAT_JOBS = 2 * MAX_VIEWS;       // (slot, view) pairs
+constexpr uint32_t SPLAT_BLOCK = 256;               // source pixels per K1 block
+constexpr uint32_t TILE_W = 16, TILE_H = 16;        // resolve tile (one block)
+
+// One source view of the reproject+splat stage (K1, renderer.cu:76-146).
+struct SplatJob {
+    const float *z;
+    const float *mx, *my, *mz;  // null: no object motion
+    float4 *full_ff;
+    float4 *cam_ff;
+    unsigned long long *splat;
+    uint32_t w, h, n_px, block_begin;
+    float weight;
+    int32_t main_slot;  // which main view's VP decides visible_in_main
+    int32_t pad0, pad1;
+    Mat4 inv_vp_from;
+    Mat4 vp_to;
+};
+
+struct SplatArgs {
+    SplatJob job[MAX_SPLAT_JOBS];
+    Mat4 vp_cam;
+    Mat4 vp_main[2];
+    int32_t n_jobs;
+    int32_t init_linear;
+    uint32_t n_blocks;
+    int32_t pad;
+};
+
+// Buffers of one (slot, view) read by the resolve stage (CamReprojInfo,
+// include/renderer.h:37-50).
+struct ResolveView {
+    const uchar4 *img;
+    const float *z;
+    const uchar4 *sh_info;
+    const uchar4 *sh_a;
+    const uchar4 *sh_b;
+    const float4 *cam_ff;
+    const float4 *full_ff;
+    const unsigned long long *splat;
+    uint32_t w, h;
+    int32_t priority;
+    int32_t forward;
+};
+
+struct ResolveArgs {
+    ResolveView fwd[MAX_VIEWS];
+    ResolveView bwd[MAX_VIEWS];
+    uchar4 *out;
+    uint32_t out_pitch_px;
+    uint32_t w, h;
+    float alpha;
+    int32_t fpa, sfpa, no_shadows, n_aux, max_iter;
+    uint32_t tiles_x, n_tiles;
+};
+
+struct ClearArgs {
+    unsigned long long *buf[MAX_SPLAT_JOBS];
+    uint32_t n[MAX_SPLAT_JOBS];
+    uint32_t block_begin[MAX_SPLAT_JOBS];
+    int32_t n_bufs;
+    uint32_t n_blocks;
+};
+
+struct TimewarpArgs {
+    Mat4 inv_vp_from;   // render camera
+    Mat4 inv_vp_to;     // fwd keyframe
+    Mat4 vp_to;
+    Mat4 vp_aux;
+    const float *depth_to;  // device scalar: mean z of the fwd keyframe
+    const uchar4 *img_to;
+    const uchar4 *img_aux;
+    uchar4 *out;
+    uint32_t out_pitch_px;
+    uint32_t w, h, w_to, h_to, w_aux, h_aux;
+    int32_t aux_set;
+    uint32_t tiles_x, n_tiles;
+};
+
+hipError_t launch_clear(const ClearArgs &a, hipStream_t s);
+hipError_t launch_splat(const SplatArgs &a, hipStream_t s);
+hipError_t launch_resolve(const ResolveArgs &a, int window, bool always_valid, hipStream_t s);
+hipError_t launch_mean_z(const float *z, uint32_t w, uint32_t h, double *row_sums, float *mean, hipStream_t s);
+hipError_t launch_timewarp(const TimewarpArgs &a, hipStream_t s);
+
+}  // namespace isti

@@ -1,0 +1,179 @@
+/*
+ * isti.h - C ABI of the MI355X (gfx950) per-frame interpolation renderer.
+ *
+ * This is the drop-in boundary for the reference's `Renderer::render` path
+ * (include/renderer.h:52-56, src/renderer.cu:607-694). The reference binds a
+ * C++ class directly; here the same call shape is exposed as plain C so a
+ * C++ wrapper (isti_renderer.hpp), ctypes, or any FFI can bind it. Nothing in
+ * these signatures is a torch or HIP C++ type: device buffers are plain
+ * pointers, the stream is an opaque `void*` (a hipStream_t, or NULL for the
+ * default stream).
+ *
+ * Conventions (fixed by the reference, restated here so this header stands on
+ * its own):
+ *   - matrices are 4x4 float, column-major, element (row r, col c) at m[c*4+r]
+ *     (glm::mat4 layout, include/data.h:114-117);
+ *   - pixel (x, y) maps to NDC (2(x+.5)/W-1, 2(y+.5)/H-1)
+ *     (include/util/helper_common.h:66-75);
+ *   - depth planes hold OpenGL NDC z in [-1, 1] (src/file_reader.cu:17-37);
+ *   - object motion is a clip-space delta in the keyframe's own camera that
+ *     moves the pixel's surface point to its position at the other keyframe
+ *     (src/renderer.cu:101-102);
+ *   - colour planes are RGBA8, rows packed (pitch = width * 4 bytes);
+ *   - shadow planes follow the reference's soft-shadow encoding
+ *     (src/renderer.cu:217-295): `shadow_info` RGB is a 24-bit field
+ *     (R<<16 | G<<8 | B) holding a 2-bit shadow code per sub-frame,
+ *     `shadow_color_a` / `shadow_color_b` the colours at the minimum / maximum
+ *     code.
+ *
+ * Errors: every entry point returns an isti_status; no C++ exception crosses
+ * this boundary. `isti_last_error` returns the message of the last failure of
+ * a renderer (or of isti_create when passed NULL). The reference prints and
+ * swallows device errors (include/util/cuda_helper_host.h:19-42) and throws
+ * std::runtime_error on shape changes (src/util/image_buffer.cu:44); both
+ * become status codes here.
+ */
+#ifndef ISTI_H
+#define ISTI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Largest number of auxiliary cameras one renderer accepts. */
+#define ISTI_MAX_AUX 4
+/* Largest view width/height: source pixels are packed as 16-bit coordinates
+ * into the splat keys (src/renderer.cu:65-68). */
+#define ISTI_MAX_DIM 65535
+/* Largest shadow_frames_per_anchor: 2-bit codes in a 24-bit field. */
+#define ISTI_MAX_SHADOW_FRAMES 12
+
+typedef enum isti_status {
+    ISTI_OK = 0,
+    ISTI_ERR_INVALID_ARG = 1,
+    ISTI_ERR_OOM = 2,
+    ISTI_ERR_DEVICE = 3,
+    ISTI_ERR_UNSUPPORTED_SHAPE = 4
+} isti_status;
+
+/* RenderMode, include/renderer.h:9-16. */
+typedef enum isti_render_mode {
+    ISTI_MODE_OURS = 0,
+    ISTI_MODE_BACKWARD_BIDIRECTIONAL = 1,
+    ISTI_MODE_BACKWARD_UNIDIRECTIONAL = 2,
+    ISTI_MODE_SPLATTING = 3,
+    ISTI_MODE_TIMEWARP = 4
+} isti_render_mode;
+
+/* Keyframe slots: FrameData::fwd / FrameData::bwd (include/data.h:179-180),
+ * and for aux cameras AuxCamData::curr / ::next (include/data.h:167). */
+typedef enum isti_slot {
+    ISTI_SLOT_FWD = 0,
+    ISTI_SLOT_BWD = 1
+} isti_slot;
+
+/* One keyframe view (MotionZBufferImage, include/data.h:12-52). All plane
+ * pointers are device pointers owned by the caller and must stay valid while
+ * the renderer may read them (until the slot is set again or the renderer is
+ * destroyed). View 0 is the main camera, views 1..n_aux the aux cameras.
+ *   motion_x/y/z: all three NULL means "no object motion" (the reference's
+ *                 std::nullopt object_motion, src/renderer.cu:539-546);
+ *   shadow_*:     each may be NULL (the reference's texture handle -1,
+ *                 src/renderer.cu:652). */
+typedef struct isti_view_desc {
+    uint32_t width;
+    uint32_t height;
+    const uint8_t *color;          /* RGBA8 */
+    const float *z;                /* NDC z */
+    const float *motion_x;         /* clip-space motion, x */
+    const float *motion_y;         /* clip-space motion, y */
+    const float *motion_z;         /* clip-space motion, z */
+    const uint8_t *shadow_info;    /* RGBA8, RGB = 24-bit code field */
+    const uint8_t *shadow_color_a; /* RGBA8, colour at the minimum code */
+    const uint8_t *shadow_color_b; /* RGBA8, colour at the maximum code */
+    float viewproj[16];
+    float inv_viewproj[16];
+    int32_t priority;              /* aux camera priority (src/file_reader.cu:102-104) */
+} isti_view_desc;
+
+/* RenderInfo + AdditionalRenderInfo (include/renderer.h:18-35).
+ * max_search_iterations extends the reference: < 0 selects the mode's own
+ * count (modes 0/2: 3, mode 1: 5, mode 3: 0, src/renderer.cu:678-689);
+ * a value >= 0 caps the fixed-point search at that many steps (it stops
+ * early at a fixed point, which is bit-identical to running all steps). */
+typedef struct isti_render_info {
+    uint32_t width;
+    uint32_t height;
+    float alpha;
+    float viewproj[16];
+    float inv_viewproj[16];
+    int32_t frames_per_anchor;
+    int32_t shadow_frames_per_anchor;
+    int32_t no_shadows;
+    int32_t mode;
+    int32_t max_search_iterations;
+} isti_render_info;
+
+typedef struct isti_config {
+    int32_t device; /* HIP device ordinal */
+    int32_t n_aux;  /* number of aux cameras, 0..ISTI_MAX_AUX */
+} isti_config;
+
+typedef struct isti_renderer isti_renderer;
+
+/* Device scratch of one (slot, view) after the last render, for inspection:
+ * the u64 splat buffer and the two float4 flow planes (cam_ff: source ->
+ * target camera, full_ff: source -> other keyframe), all device pointers,
+ * rows packed. Reference: Renderer members, include/renderer.h:64-92. */
+typedef struct isti_scratch_view {
+    uint32_t width;
+    uint32_t height;
+    const uint64_t *splat;
+    const float *cam_ff;
+    const float *full_ff;
+} isti_scratch_view;
+
+/* Renderer::Renderer (include/renderer.h:54). Creates a renderer bound to
+ * config->device. Allocates nothing on the device until the first render. */
+isti_status isti_create(const isti_config *config, isti_renderer **out);
+
+/* Frees all device scratch (the reference has no destructor and leaks). */
+isti_status isti_destroy(isti_renderer *r);
+
+/* Sets one keyframe view of a slot (the reference loads FrameData through
+ * FrameFileReader, src/file_reader.cu:138-217; here the caller hands over
+ * device planes). view: 0 = main, 1..n_aux = aux camera view-1. */
+isti_status isti_keyframe_set(isti_renderer *r, int32_t slot, int32_t view,
+                              const isti_view_desc *desc);
+
+/* FrameData::swap (include/data.h:110-116): exchanges the fwd and bwd
+ * descriptors of the main view and of every aux camera. */
+isti_status isti_keyframe_swap(isti_renderer *r);
+
+/* Renderer::render (src/renderer.cu:607-694). Writes RGBA8 pixels into the
+ * caller-owned device image `out_rgba8` (row pitch `out_pitch_bytes`) with the
+ * reference's contract: only pixels with a valid result are written, so holes
+ * keep the previous content (the reference's hole fill). Stream-ordered on
+ * `stream`; no host synchronisation inside. */
+isti_status isti_render(isti_renderer *r, const isti_render_info *info,
+                        uint8_t *out_rgba8, size_t out_pitch_bytes, void *stream);
+
+/* Device scratch of (slot, view) from the last render (see above). */
+isti_status isti_scratch(const isti_renderer *r, int32_t slot, int32_t view,
+                         isti_scratch_view *out);
+
+/* Message of the last failure of `r`, or of the calling thread's last
+ * isti_create failure when r is NULL. Never NULL. */
+const char *isti_last_error(const isti_renderer *r);
+
+/* Library version string. */
+const char *isti_version(void);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* ISTI_H */
