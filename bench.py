@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""Benchmark of the per-frame interpolation path on one MI355X.
+
+Metric: interpolated Mpix/s and p99 frame latency at 1080p. Workload: the
+1920x1080 synthetic scene with two aux cameras (960x540, +-0.3 m) and soft
+shadows (area light r=0.5 m, 16 samples, 8 shadow sub-frames); keyframes 8
+client frames apart, so one step interpolates the 7 intermediate frames of a
+keyframe pair (alpha = 1/8 .. 7/8), mode 0 with the fixed-point search
+capped at 32 iterations, fp32 buffers. Inputs are resident in HBM before the
+timed region starts.
+
+Prints ONE JSON line. `roofline` covers the dominant kernel: its algorithmic
+bytes per launch divided by its average launch time, measured with HIP
+events recorded by the renderer on its own stream across the timed region.
+`cpu_baseline` times the CPU oracle (the C++ restatement of the reference
+algorithm) on a bounded sample of the same frames.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "interpolated Mpix/s & p99 frame ms at 1080p, learned-init vs plain search"
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=200)
+    p.add_argument("--warmup", type=int, default=20)
+    p.add_argument("--seed", type=int, default=1)
+    p.add_argument("--iters", type=int, default=32, help="fixed-point search cap")
+    p.add_argument("--mode", type=int, default=0)
+    p.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-baseline sample budget")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--traffic-file", default=str(ROOT / "profiles" / "traffic_resolve_1080p.json"))
+    return p.parse_args()
+
+
+def algorithmic_bytes(src_px: int, out_px: int, shadows: bool) -> dict:
+    """Compulsory bytes per launch of each stage.
+
+    reproject+splat: z 4 + motion 12 read, one 8-byte splat write per source pixel.
+    resolve: splat 8 + colour 4 (+ shadow info 4) read per source pixel, 4 B written per output pixel.
+    clear: 8 B written per splat cell.
+    """
+    s = 4 if shadows else 0
+    return {
+        "clear": 8 * src_px,
+        "splat": 24 * src_px,
+        "resolve": (12 + s) * src_px + 4 * out_px,
+    }
+
+
+def cpu_baseline(fwd, bwd, infos, budget_s: float):
+    from oracle import oracle as O
+
+    threads = min(16, os.cpu_count() or 1)
+    h, w = fwd[0]["z"].shape
+    out = np.zeros((h, w, 4), np.uint8)
+    pr = [0] + [3] * (len(fwd) - 1)
+    done, t0 = 0, time.perf_counter()
+    for info in infos:
+        O.render(fwd, bwd, info, out, priorities=pr, threads=threads)
+        done += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(done * w * h / dt / 1e6, 3),
+        "unit": "Mpix/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{done} of the 7 intermediate 1920x1080 frames of one keyframe pair (same inputs), "
+                  f"oracle with {threads} OpenMP threads, {dt:.1f} s",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); every run of this path "
+              "is a single-GPU workload, see DESIGN.md for how it would shard", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from imagebasedspatiotemporalinterpolation_amd import _native as N
+    from imagebasedspatiotemporalinterpolation_amd import renderer as R
+    from imagebasedspatiotemporalinterpolation_amd import scene
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    cfg = scene.defaults(2, args.seed)
+    fpa, sfpa = cfg.keyframe_spacing, cfg.shadow_frames
+    views = list(range(1 + cfg.n_aux))
+    t0 = time.perf_counter()
+    fwd = [scene.keyframe(cfg, 0, v, +1) for v in views]
+    bwd = [scene.keyframe(cfg, 1, v, -1) for v in views]
+    print(f"[bench] generated keyframes in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+
+    kv_f = [R.KeyframeView.from_numpy(v) for v in fwd]
+    kv_b = [R.KeyframeView.from_numpy(v) for v in bwd]
+    fd = R.FrameData(kv_f[0], kv_b[0], [R.AuxCamData(kv_f[i], kv_b[i], 3) for i in views[1:]])
+    W, H = cfg.width, cfg.height
+    rend = R.Renderer(n_aux=cfg.n_aux)
+    rend.set_frame(fd)
+    out = torch.zeros((H, W, 4), dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+    L = N.lib()
+    infos = []
+    for b in range(1, fpa):
+        vp, ivp, _, _ = scene.camera(cfg, b, 0)
+        ri = R.RenderInfo((W, H), b / fpa, vp, ivp, R.AdditionalRenderInfo(fpa, sfpa, False, R.RenderMode(args.mode)),
+                          args.iters)
+        infos.append(ri.to_c())
+    h, sp, pitch, optr = rend._h, ctypes.c_void_p(stream.cuda_stream), W * 4, out.data_ptr()
+
+    def frame(ci):
+        st = L.isti_render(h, ctypes.byref(ci), optr, pitch, sp)
+        if st != 0:
+            N.check(st, h)
+
+    for _ in range(args.warmup):
+        for ci in infos:
+            frame(ci)
+    torch.cuda.synchronize()
+
+    n_frames = args.steps * len(infos)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n_frames)]
+    rend.set_stage_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    k = 0
+    for _ in range(args.steps):
+        for ci in infos:
+            ev[k][0].record(stream)
+            frame(ci)
+            ev[k][1].record(stream)
+            k += 1
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    totals, count = rend.stage_totals()
+    rend.set_stage_timing(False)
+    frame_ms = np.array([a.elapsed_time(b) for a, b in ev])
+
+    src_px = sum(int(v["z"].size) for v in fwd) + sum(int(v["z"].size) for v in bwd)
+    out_px = W * H
+    alg = algorithmic_bytes(src_px, out_px, bool(cfg.shadows))
+    stage_avg_ms = {name: totals[i] / max(count, 1) for i, name in enumerate(("clear", "splat", "resolve"))}
+    dom = max(stage_avg_ms, key=stage_avg_ms.get)
+    achieved = alg[dom] / (stage_avg_ms[dom] * 1e-3) / 1e9
+    traffic = None
+    tf = Path(args.traffic_file)
+    if tf.exists():
+        try:
+            traffic = json.loads(tf.read_text()).get(dom)
+        except (ValueError, OSError):
+            traffic = None
+
+    value = n_frames * out_px / wall / 1e6
+    res = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded analytic ray-cast scene, generated on the host, resident in HBM)",
+        "config": {
+            "workload": "1920x1080 main + 2 aux 960x540 (+-0.3 m), soft shadows (8 sub-frames), "
+                        "7 interpolated frames per keyframe pair (fpa 8), mode 0, plain search <= 32 iterations",
+            "frames_per_step": len(infos),
+            "search": "plain",
+            "max_search_iterations": args.iters,
+            "mode": args.mode,
+        },
+        "p50_frame_ms": round(float(np.percentile(frame_ms, 50)), 4),
+        "p99_frame_ms": round(float(np.percentile(frame_ms, 99)), 4),
+        "stage_avg_ms": {k2: round(v2, 5) for k2, v2 in stage_avg_ms.items()},
+        "learned_init": None,
+        "roofline": {
+            "kernel": dom,
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "algorithmic_bytes": alg[dom],
+            "traffic": traffic,
+        },
+    }
+    if not args.no_cpu_baseline:
+        print("[bench] timing the CPU oracle sample", file=sys.stderr, flush=True)
+        res["cpu_baseline"] = cpu_baseline(fwd, bwd, infos, args.cpu_seconds)
+    else:
+        res["cpu_baseline"] = None
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -30,6 +30,9 @@ EXPORTED_SYMBOLS = (
     "isti_keyframe_swap",
     "isti_render",
     "isti_scratch",
+    "isti_set_stage_timing",
+    "isti_stage_times",
+    "isti_stage_totals",
     "isti_last_error",
     "isti_version",
 )
@@ -121,6 +124,12 @@ def lib() -> ctypes.CDLL:
     L.isti_render.restype = c_int32
     L.isti_scratch.argtypes = [c_void_p, c_int32, c_int32, POINTER(ScratchView)]
     L.isti_scratch.restype = c_int32
+    L.isti_set_stage_timing.argtypes = [c_void_p, c_int32]
+    L.isti_set_stage_timing.restype = c_int32
+    L.isti_stage_times.argtypes = [c_void_p, POINTER(c_float)]
+    L.isti_stage_times.restype = c_int32
+    L.isti_stage_totals.argtypes = [c_void_p, POINTER(ctypes.c_double), POINTER(ctypes.c_int64)]
+    L.isti_stage_totals.restype = c_int32
     L.isti_last_error.argtypes = [c_void_p]
     L.isti_last_error.restype = c_char_p
     L.isti_version.argtypes = []

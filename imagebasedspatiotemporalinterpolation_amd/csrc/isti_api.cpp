@@ -31,6 +31,19 @@ struct Scratch {
     float4 *full_ff = nullptr;
 };
 
+// Ring of per-render event sets for stage timing. A slot is harvested (its
+// durations added to the totals) when it is about to be reused, by which
+// time its render has long finished.
+constexpr int RING = 64;
+struct TimingRing {
+    hipEvent_t ev[RING][4] = {};
+    bool pending[RING] = {};
+    int head = 0;
+    int last = -1;
+    double total[3] = {0.0, 0.0, 0.0};
+    int64_t count = 0;
+};
+
 }  // namespace
 
 struct isti_renderer {
@@ -42,6 +55,8 @@ struct isti_renderer {
     double *row_sums = nullptr;
     uint32_t row_sums_cap = 0;
     float *mean_z = nullptr;
+    bool timing = false;
+    TimingRing ring;
     std::string err;
 };
 
@@ -136,6 +151,34 @@ isti_status validate(isti_renderer *r, const isti_render_info *info, size_t out_
     return ISTI_OK;
 }
 
+hipError_t harvest(TimingRing &t, int slot) {
+    if (!t.pending[slot]) return hipSuccess;
+    hipError_t e = hipEventSynchronize(t.ev[slot][3]);
+    float ms[3];
+    for (int i = 0; i < 3 && e == hipSuccess; i++) e = hipEventElapsedTime(&ms[i], t.ev[slot][i], t.ev[slot][i + 1]);
+    if (e != hipSuccess) return e;
+    for (int i = 0; i < 3; i++) t.total[i] += ms[i];
+    t.count++;
+    t.pending[slot] = false;
+    return hipSuccess;
+}
+
+// Records stage boundary `i` of the current render when timing is on.
+// Boundary 0 claims the next ring slot (harvesting it first if needed).
+hipError_t mark(isti_renderer *r, int i, hipStream_t s) {
+    if (!r->timing) return hipSuccess;
+    TimingRing &t = r->ring;
+    if (i == 0) {
+        hipError_t e = harvest(t, t.head);
+        if (e != hipSuccess) return e;
+        t.last = t.head;
+        t.head = (t.head + 1) % RING;
+    }
+    hipError_t e = hipEventRecord(t.ev[t.last][i], s);
+    if (e == hipSuccess && i == 3) t.pending[t.last] = true;
+    return e;
+}
+
 isti_status render_timewarp(isti_renderer *r, const isti_render_info *info, uint8_t *out, size_t pitch,
                             hipStream_t s) {
     // src/renderer.cu:615-629
@@ -152,8 +195,10 @@ isti_status render_timewarp(isti_renderer *r, const isti_render_info *info, uint
         hipError_t e = hipMalloc(&r->mean_z, sizeof(float));
         if (e != hipSuccess) return dev_fail(r, e, "hipMalloc(mean z)");
     }
-    hipError_t e = launch_mean_z(f.z, f.width, f.height, r->row_sums, r->mean_z, s);
+    hipError_t e = mark(r, 0, s);
+    if (e == hipSuccess) e = launch_mean_z(f.z, f.width, f.height, r->row_sums, r->mean_z, s);
     if (e != hipSuccess) return dev_fail(r, e, "mean z launch");
+    if ((e = mark(r, 1, s)) != hipSuccess || (e = mark(r, 2, s)) != hipSuccess) return dev_fail(r, e, "event");
 
     TimewarpArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -180,6 +225,7 @@ isti_status render_timewarp(isti_renderer *r, const isti_render_info *info, uint
     a.n_tiles = a.tiles_x * ((a.h + TILE_H - 1) / TILE_H);
     e = launch_timewarp(a, s);
     if (e != hipSuccess) return dev_fail(r, e, "timewarp launch");
+    if ((e = mark(r, 3, s)) != hipSuccess) return dev_fail(r, e, "event");
     return ISTI_OK;
 }
 
@@ -227,6 +273,9 @@ isti_status isti_destroy(isti_renderer *r) {
         for (auto &s : slot) free_scratch(s);
     if (r->row_sums) (void)hipFree(r->row_sums);
     if (r->mean_z) (void)hipFree(r->mean_z);
+    for (auto &set : r->ring.ev)
+        for (auto &ev : set)
+            if (ev) (void)hipEventDestroy(ev);
     delete r;
     return ISTI_OK;
 }
@@ -353,12 +402,16 @@ isti_status isti_render(isti_renderer *r, const isti_render_info *info, uint8_t 
     ra.tiles_x = (ra.w + TILE_W - 1) / TILE_W;
     ra.n_tiles = ra.tiles_x * ((ra.h + TILE_H - 1) / TILE_H);
 
+    if ((e = mark(r, 0, s)) != hipSuccess) return dev_fail(r, e, "event");
     e = launch_clear(ca, s);
     if (e != hipSuccess) return dev_fail(r, e, "clear launch");
+    if ((e = mark(r, 1, s)) != hipSuccess) return dev_fail(r, e, "event");
     e = launch_splat(sa, s);
     if (e != hipSuccess) return dev_fail(r, e, "splat launch");
+    if ((e = mark(r, 2, s)) != hipSuccess) return dev_fail(r, e, "event");
     e = launch_resolve(ra, window, always_valid, s);
     if (e != hipSuccess) return dev_fail(r, e, "resolve launch");
+    if ((e = mark(r, 3, s)) != hipSuccess) return dev_fail(r, e, "event");
     return ISTI_OK;
 }
 
@@ -370,6 +423,51 @@ isti_status isti_scratch(const isti_renderer *r, int32_t slot, int32_t view, ist
     out->splat = reinterpret_cast<const uint64_t *>(s.splat);
     out->cam_ff = reinterpret_cast<const float *>(s.cam_ff);
     out->full_ff = reinterpret_cast<const float *>(s.full_ff);
+    return ISTI_OK;
+}
+
+isti_status isti_set_stage_timing(isti_renderer *r, int32_t enable) {
+    if (!r) return ISTI_ERR_INVALID_ARG;
+    hipError_t e = hipSetDevice(r->device);
+    if (e != hipSuccess) return dev_fail(r, e, "hipSetDevice");
+    TimingRing &t = r->ring;
+    if (enable && !t.ev[0][0]) {
+        for (auto &set : t.ev)
+            for (auto &ev : set)
+                if ((e = hipEventCreate(&ev)) != hipSuccess) return dev_fail(r, e, "hipEventCreate");
+    }
+    if (enable) {
+        for (int i = 0; i < RING; i++) t.pending[i] = false;
+        t.head = 0;
+        t.last = -1;
+        t.total[0] = t.total[1] = t.total[2] = 0.0;
+        t.count = 0;
+    }
+    r->timing = enable != 0;
+    return ISTI_OK;
+}
+
+isti_status isti_stage_times(isti_renderer *r, float ms[3]) {
+    if (!r || !ms) return ISTI_ERR_INVALID_ARG;
+    TimingRing &t = r->ring;
+    if (t.last < 0 || !t.pending[t.last])
+        return fail(r, ISTI_ERR_INVALID_ARG, "no timed render pending (enable isti_set_stage_timing)");
+    hipError_t e = hipEventSynchronize(t.ev[t.last][3]);
+    for (int i = 0; i < 3 && e == hipSuccess; i++) e = hipEventElapsedTime(&ms[i], t.ev[t.last][i], t.ev[t.last][i + 1]);
+    if (e != hipSuccess) return dev_fail(r, e, "hipEventElapsedTime");
+    return ISTI_OK;
+}
+
+isti_status isti_stage_totals(isti_renderer *r, double total_ms[3], int64_t *count) {
+    if (!r || !total_ms || !count) return ISTI_ERR_INVALID_ARG;
+    TimingRing &t = r->ring;
+    for (int k = 0; k < RING; k++) {
+        const int slot = (t.head + k) % RING;  // oldest first
+        hipError_t e = harvest(t, slot);
+        if (e != hipSuccess) return dev_fail(r, e, "hipEventElapsedTime");
+    }
+    for (int i = 0; i < 3; i++) total_ms[i] = t.total[i];
+    *count = t.count;
     return ISTI_OK;
 }
 

@@ -195,6 +195,23 @@ class Renderer:
         """FrameData::swap on the renderer's own keyframe descriptors."""
         N.check(N.lib().isti_keyframe_swap(self._h), self._h)
 
+    def set_stage_timing(self, enable: bool = True) -> None:
+        """Record HIP events between the stages of every render (off by default)."""
+        N.check(N.lib().isti_set_stage_timing(self._h, 1 if enable else 0), self._h)
+
+    def stage_times(self) -> tuple:
+        """(clear, reproject+splat, resolve) GPU milliseconds of the last timed render."""
+        ms = (ctypes.c_float * 3)()
+        N.check(N.lib().isti_stage_times(self._h, ms), self._h)
+        return tuple(ms)
+
+    def stage_totals(self):
+        """((clear, splat, resolve) summed GPU ms, render count) since timing was enabled."""
+        tot = (ctypes.c_double * 3)()
+        n = ctypes.c_int64()
+        N.check(N.lib().isti_stage_totals(self._h, tot, ctypes.byref(n)), self._h)
+        return tuple(tot), n.value
+
     def scratch(self, slot: int, view: int) -> dict:
         """Host copies of (slot, view)'s splat buffer and flow planes after the last render."""
         sv = N.ScratchView()

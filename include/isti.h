@@ -165,6 +165,21 @@ isti_status isti_render(isti_renderer *r, const isti_render_info *info,
 isti_status isti_scratch(const isti_renderer *r, int32_t slot, int32_t view,
                          isti_scratch_view *out);
 
+/* Per-stage GPU timing (the reference has no timers): when enabled,
+ * isti_render records HIP events between its stages on the render stream,
+ * into a ring of event sets that is harvested lazily (no extra host sync in
+ * steady state). Enabling also resets the totals. Off by default. */
+isti_status isti_set_stage_timing(isti_renderer *r, int32_t enable);
+
+/* Stage durations of the last timed render, in milliseconds:
+ * ms[0] splat clear (mode 4: mean z), ms[1] reproject + splat (mode 4: 0),
+ * ms[2] resolve (mode 4: timewarp). Waits for that render to finish. */
+isti_status isti_stage_times(isti_renderer *r, float ms[3]);
+
+/* Sums of the stage durations (ms, same order) over every timed render
+ * since timing was enabled, and their count. Waits for pending renders. */
+isti_status isti_stage_totals(isti_renderer *r, double total_ms[3], int64_t *count);
+
 /* Message of the last failure of `r`, or of the calling thread's last
  * isti_create failure when r is NULL. Never NULL. */
 const char *isti_last_error(const isti_renderer *r);

@@ -1,0 +1,134 @@
+"""Parity of the HIP path (through the C ABI) against the CPU oracle.
+
+Bar (per case): splat winner buffers bit-exact; cam_ff / full_ff within
+1e-3 px; final RGBA within 1/255 max-abs and >= 50 dB PSNR (RGB, as
+scripts/eval.py:57,68 of the reference). Every case starts from a random
+previous output, so hole retention (write-only-if-valid, renderer.cu:416-419,
+453-464) is checked too.
+"""
+import numpy as np
+import pytest
+
+from tests import _frames as F
+
+pytestmark = pytest.mark.gpu
+
+
+def _prev(h, w, seed=11):
+    return np.random.default_rng(seed).integers(0, 256, (h, w, 4), dtype=np.uint8)
+
+
+def _check(O, cfg, fwd, bwd, b, mode=0, max_iter=-1, no_shadows=False, alpha=None, k=0):
+    fpa = cfg.keyframe_spacing
+    sfpa = cfg.shadow_frames if cfg.shadows else 0
+    vp, ivp, w, h = F.target(cfg, k, b)
+    a = b / fpa if alpha is None else alpha
+    prev = _prev(h, w)
+    print(f"case: {w}x{h} n_aux={len(fwd)-1} mode={mode} alpha={a} iters={max_iter}", flush=True)
+    ref, sref = F.oracle_frame(O, fwd, bwd, vp, ivp, a, fpa, sfpa, mode, max_iter, no_shadows, prev)
+    got, sgot = F.gpu_frame(fwd, bwd, vp, ivp, a, fpa, sfpa, mode, max_iter, no_shadows, prev)
+    if mode != 4:
+        for key, s in sref.items():
+            g = sgot[key]
+            np.testing.assert_array_equal(g["splat"], s["splat"], err_msg=f"splat {key}")
+            assert F.flow_err_px(g["cam_ff"], s["cam_ff"]) <= 1e-3, f"cam_ff {key}"
+            assert F.flow_err_px(g["full_ff"], s["full_ff"]) <= 1e-3, f"full_ff {key}"
+            np.testing.assert_array_equal(g["cam_ff"][..., 3], s["cam_ff"][..., 3])
+    F.assert_rgb_parity(got, ref)
+    return got, ref
+
+
+@pytest.fixture(scope="module")
+def c1():
+    cfg = F.scaled(1, 5)
+    return cfg, F.pair(cfg, 0)
+
+
+@pytest.mark.parametrize("b", [1, 2, 3])
+def test_c1_mode0(oracle_lib, c1, b):
+    cfg, (fwd, bwd) = c1
+    _check(oracle_lib, cfg, fwd, bwd, b)
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3, 4])
+def test_c1_other_modes(oracle_lib, c1, mode):
+    cfg, (fwd, bwd) = c1
+    _check(oracle_lib, cfg, fwd, bwd, 2, mode=mode)
+
+
+def test_c1_late_keyframe_extrapolation(oracle_lib, c1):
+    """alpha > 1 (latency, main.cu:157): fwd search skipped, no blending."""
+    cfg, (fwd, bwd) = c1
+    _check(oracle_lib, cfg, fwd, bwd, 1, alpha=1.25)
+
+
+def test_c1_no_object_motion(oracle_lib, c1):
+    cfg, (fwd, bwd) = c1
+    fwd2 = [F.strip(v, motion=False) for v in fwd]
+    bwd2 = [F.strip(v, motion=False) for v in bwd]
+    _check(oracle_lib, cfg, fwd2, bwd2, 2)
+
+
+def test_c1_search_cap_32(oracle_lib, c1):
+    cfg, (fwd, bwd) = c1
+    _check(oracle_lib, cfg, fwd, bwd, 3, max_iter=32)
+
+
+@pytest.fixture(scope="module")
+def c2_small():
+    cfg = F.scaled(2, 9, div=4)  # 480x270, aux 240x135, shadows on
+    return cfg, F.pair(cfg, 0)
+
+
+@pytest.mark.parametrize("b", [1, 3, 4, 7])
+def test_c2_small_shadows_aux(oracle_lib, c2_small, b):
+    cfg, (fwd, bwd) = c2_small
+    _check(oracle_lib, cfg, fwd, bwd, b, max_iter=32)
+
+
+def test_c2_small_no_shadows_flag(oracle_lib, c2_small):
+    cfg, (fwd, bwd) = c2_small
+    _check(oracle_lib, cfg, fwd, bwd, 5, max_iter=32, no_shadows=True)
+
+
+def test_c2_small_missing_shadow_planes(oracle_lib, c2_small):
+    """Absent planes read as the reference's texture handle -1 (renderer.cu:277)."""
+    cfg, (fwd, bwd) = c2_small
+    fwd2 = [F.strip(v, shadows=False) for v in fwd]
+    bwd2 = [dict(v, shadow_color_b=None) for v in bwd]
+    _check(oracle_lib, cfg, fwd2, bwd2, 3, max_iter=32)
+
+
+@pytest.mark.parametrize("mode", [1, 3, 4])
+def test_c2_small_modes(oracle_lib, c2_small, mode):
+    cfg, (fwd, bwd) = c2_small
+    _check(oracle_lib, cfg, fwd, bwd, 2, mode=mode)
+
+
+def test_c3_small_stereo_eye(oracle_lib):
+    cfg = F.scaled(3, 4, div=4)  # 516x552 per eye, 64 spheres on sinusoidal paths
+    for eye in (0, 1):
+        fwd, bwd = F.pair(cfg, 0, eye)
+        fpa = cfg.keyframe_spacing
+        vp, ivp, w, h = F.target(cfg, 0, 3, eye)
+        prev = _prev(h, w)
+        ref, _ = F.oracle_frame(oracle_lib, fwd, bwd, vp, ivp, 0.5, fpa, 0, prev=prev)
+        got, _ = F.gpu_frame(fwd, bwd, vp, ivp, 0.5, fpa, 0, prev=prev, want_scratch=False)
+        F.assert_rgb_parity(got, ref)
+
+
+def test_c2_full_1080p(oracle_lib):
+    """Full-size workload: 1920x1080, 2 aux 960x540, shadows, search cap 32."""
+    cfg = F.scaled(2, 3)
+    fwd, bwd = F.pair(cfg, 0)
+    _check(oracle_lib, cfg, fwd, bwd, 4, max_iter=32)
+
+
+def test_repeat_run_bit_identical(c1):
+    """K1's atomicMin is order-independent: two renders give identical bytes."""
+    cfg, (fwd, bwd) = c1
+    vp, ivp, w, h = F.target(cfg, 0, 2)
+    a, sa = F.gpu_frame(fwd, bwd, vp, ivp, 0.5, 4, 0)
+    b, sb = F.gpu_frame(fwd, bwd, vp, ivp, 0.5, 4, 0)
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(sa[(0, 0)]["splat"], sb[(0, 0)]["splat"])
