@@ -117,6 +117,11 @@ ResolveView resolve_view(const isti_view_desc &d, const Scratch &s, bool forward
     v.h = d.height;
     v.priority = priority;
     v.forward = forward ? 1 : 0;
+    v.inv_w = 1.0 / (double)d.width;
+    v.inv_h = 1.0 / (double)d.height;
+    const float mn = (float)(d.width < d.height ? d.width : d.height);
+    v.rs_x = (float)d.width / mn;
+    v.rs_y = (float)d.height / mn;
     return v;
 }
 
@@ -362,6 +367,8 @@ isti_status isti_render(isti_renderer *r, const isti_render_info *info, uint8_t 
             splat_blocks += (n + SPLAT_BLOCK - 1) / SPLAT_BLOCK;
             J.weight = sl == 0 ? info->alpha : 1.0f - info->alpha;
             J.main_slot = sl;
+            J.inv_w = 1.0 / (double)d.width;
+            J.inv_h = 1.0 / (double)d.height;
             J.inv_vp_from = to_mat(d.inv_viewproj);
             J.vp_to = to_mat(other.viewproj);
             nj++;

@@ -31,24 +31,24 @@ constexpr unsigned long long SPLAT_EMPTY = 0xfefefefefefefefeull;  // memset(0xf
 constexpr uint32_t SOFTSHADOW_BITS = 2;
 constexpr uint32_t SOFTSHADOW_VAL_MASK = (1u << SOFTSHADOW_BITS) - 1u;
 
+// float -> int32 with truncation, NaN -> 0, saturation: exactly what
+// v_cvt_i32_f32 does, written as the instruction so the compiler cannot
+// treat out-of-range values as undefined.
 __device__ __forceinline__ int32_t cvt_s32(float f) {
-    if (f != f) return 0;
-    if (f >= 2147483648.0f) return INT32_MAX;
-    if (f <= -2147483648.0f) return INT32_MIN;
-    return (int32_t)f;
+    int32_t r;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(f));
+    return r;
+}
+// float -> uint32 with truncation, NaN and negatives -> 0, saturation (v_cvt_u32_f32)
+__device__ __forceinline__ uint32_t sat_u32(float f) {
+    uint32_t r;
+    asm("v_cvt_u32_f32 %0, %1" : "=v"(r) : "v"(f));
+    return r;
 }
 // (unsigned char)(float) of make_uchar4(float4), helper_math_ext.h:19-22
 __device__ __forceinline__ uint8_t cvt_u8(float f) {
-    if (f != f) return 0;
-    if (f <= 0.0f) return 0;
-    if (f >= 255.0f) return 255;
-    return (uint8_t)f;
-}
-__device__ __forceinline__ uint32_t sat_u32(float r) {
-    if (r != r) return 0;
-    if (r <= 0.0f) return 0;
-    if (r >= 4294967296.0f) return 0xffffffffu;
-    return (uint32_t)r;
+    const uint32_t u = sat_u32(f);
+    return (uint8_t)(u < 255u ? u : 255u);
 }
 // __float2uint_rn / _rd / _ru, renderer.cu:359-362
 __device__ __forceinline__ uint32_t f2u_rn(float f) { return sat_u32(__builtin_rintf(f)); }
@@ -66,6 +66,16 @@ __device__ __forceinline__ float2 pixel2cs(uint32_t x, uint32_t y, uint32_t w, u
     r.y = 2.0f * ((float)y + 0.5f) / (float)h - 1.0f;
     return r;
 }
+// One coordinate of pixel2CS, RN32(RN32(2*(p+0.5)) / n) - 1, with the
+// division evaluated as RN32(RN64((2p+1) * RN64(1/n))). For 2p+1 < 2^17 and
+// n < 2^16 the exact quotient is never a binary32 rounding midpoint and lies
+// at least 2^-41 (relative) from every midpoint, while the double product is
+// within 2^-52 of it, so the result is bit-identical to the IEEE division.
+// inv_n = RN64(1/n) is computed once per view on the host.
+__device__ __forceinline__ float cs_coord(uint32_t p, double inv_n) {
+    return (float)((double)(2u * p + 1u) * inv_n) - 1.0f;
+}
+
 // helper_common.h:71-75
 __device__ __forceinline__ int2 cs2pixel(float cx, float cy, uint32_t w, uint32_t h) {
     int2 r;
@@ -82,11 +92,13 @@ __device__ __forceinline__ float4 mat_vec(const Mat4 &M, float4 v) {
     r.w = (m[3] * v.x + m[7] * v.y) + (m[11] * v.z + m[15] * v.w);
     return r;
 }
-// helper_common.h:77-87 CS2WS / WS2CS
+// helper_common.h:77-87 CS2WS / WS2CS. w/w is 1 for finite non-zero w and
+// NaN otherwise, so that quotient is a select instead of a division.
 __device__ __forceinline__ float4 project(const Mat4 &M, float4 v) {
     const float4 r = mat_vec(M, v);
     const float w = r.w;
-    return make_float4(r.x / w, r.y / w, r.z / w, r.w / w);
+    const float ww = (__builtin_isfinite(w) && w != 0.0f) ? 1.0f : __builtin_nanf("");
+    return make_float4(r.x / w, r.y / w, r.z / w, ww);
 }
 // helper_common.h:89-95
 __device__ __forceinline__ float z_to_depth(float z) { return (z + 1.0f) * 0.5f; }
@@ -111,12 +123,19 @@ __device__ __forceinline__ float key_depth(unsigned long long k) { return __uint
 __device__ __forceinline__ uint32_t key_x(unsigned long long k) { return (uint32_t)k >> 16; }
 __device__ __forceinline__ uint32_t key_y(unsigned long long k) { return (uint32_t)(k & 0xFFFFull); }
 
+// Unchecked load at pixel (x, y) of a plane with row length w (32-bit index:
+// a view has fewer than 2^32 pixels).
+template <typename T>
+__device__ __forceinline__ T at(const T *plane, uint32_t w, uint32_t x, uint32_t y) {
+    return plane[y * w + x];
+}
+
 // Point sample with border addressing (src/util/image_buffer.cu:227-249):
 // out of range, or a missing plane, reads as zero.
 template <typename T>
 __device__ __forceinline__ T fetch(const T *plane, uint32_t w, uint32_t h, uint32_t x, uint32_t y) {
     if (plane == nullptr || x >= w || y >= h) return T{};
-    return plane[(size_t)y * w + x];
+    return plane[y * w + x];
 }
 
 }  // namespace isti

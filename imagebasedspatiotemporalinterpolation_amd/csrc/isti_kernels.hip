@@ -77,8 +77,7 @@ __global__ __launch_bounds__(256) void reproject_splat_kernel(const SplatArgs a)
     float4 om = make_float4(0.f, 0.f, 0.f, 0.f);
     if (J.mx != nullptr) om = make_float4(J.mx[p], J.my[p], J.mz[p], 0.f);
 
-    const float2 pcs = pixel2cs(x, y, J.w, J.h);
-    const float4 cs_from = make_float4(pcs.x, pcs.y, z_from, 1.0f);
+    const float4 cs_from = make_float4(cs_coord(x, J.inv_w), cs_coord(y, J.inv_h), z_from, 1.0f);
     const float4 ws_from = project(J.inv_vp_from, cs_from);
     const float4 cs_moved = make_float4(cs_from.x + om.x, cs_from.y + om.y, cs_from.z + om.z, cs_from.w + 0.0f);
     const float4 ws_to = project(J.inv_vp_from, cs_moved);
@@ -110,7 +109,7 @@ __global__ __launch_bounds__(256) void reproject_splat_kernel(const SplatArgs a)
     } else {
         J.cam_ff[p] = make_float4(fx * wt, fy * wt, fz * wt, vis);
     }
-    atomicMin(&J.splat[(size_t)ty * J.w + tx], encode_key(depth, x, y));
+    atomicMin(&J.splat[ty * J.w + tx], encode_key(depth, x, y));
 }
 
 // ---------------------------------------------------------------------------
@@ -121,18 +120,19 @@ struct SplatInfo {
     uint32_t x, y;
 };
 
-// initializeSearchFromSplatting<W>, renderer.cu:148-175
+// initializeSearchFromSplatting<W>, renderer.cu:148-175, over splat cells
+// staged in LDS (row length rw; (lx, ly) is the centre). Cells outside the
+// view were staged as the empty key, whose negative depth never wins, which
+// is the reference's `continue` for out-of-range cells.
 template <int W>
-__device__ __forceinline__ SplatInfo init_search(uint32_t ix, uint32_t iy, const ResolveView &v) {
-    const unsigned long long c = v.splat[(size_t)iy * v.w + ix];
+__device__ __forceinline__ SplatInfo init_search_lds(const unsigned long long *t, int rw, int lx, int ly) {
+    const unsigned long long c = t[ly * rw + lx];
     SplatInfo best{key_depth(c), key_x(c), key_y(c)};
 #pragma unroll
     for (int dy = -W; dy <= W; dy++) {
 #pragma unroll
         for (int dx = -W; dx <= W; dx++) {
-            const int2 n = make_int2((int)ix + dx, (int)iy + dy);
-            if (!in_range(n, v.w, v.h)) continue;
-            const unsigned long long k = v.splat[(size_t)n.y * v.w + (uint32_t)n.x];
+            const unsigned long long k = t[(ly + dy) * rw + lx + dx];
             const float d = key_depth(k);
             if (d > 0.0f && (!__builtin_isfinite(best.depth) || best.depth < 0.0f || d < best.depth))
                 best = SplatInfo{d, key_x(k), key_y(k)};
@@ -141,33 +141,83 @@ __device__ __forceinline__ SplatInfo init_search(uint32_t ix, uint32_t iy, const
     return best;
 }
 
-// iterativeSearch<N>, renderer.cu:177-202. The reference always runs N
-// steps; an unchanged or out-of-range candidate is a fixed point (every later
-// step repeats it), so stopping there is bit-identical. Returns the cam_ff
-// sample at the final pixel in `m`.
-__device__ __forceinline__ uint2 search(uint2 start, float2 pcs_full, const ResolveView &v, int n_iter, float4 &m) {
-    uint2 r = start;
-    m = fetch(v.cam_ff, v.w, v.h, r.x, r.y);
-    for (int i = 0; i < n_iter; i++) {
-        const int2 t = cs2pixel(pcs_full.x - m.x, pcs_full.y - m.y, v.w, v.h);
-        if (!in_range(t, v.w, v.h)) break;
-        const uint2 nt = make_uint2((uint32_t)t.x, (uint32_t)t.y);
-        if (nt.x == r.x && nt.y == r.y) break;
-        r = nt;
-        m = fetch(v.cam_ff, v.w, v.h, r.x, r.y);
+// Stages the splat cells [ox, ox+rw) x [oy, oy+rh) of view v into LDS.
+__device__ __forceinline__ void stage_splat(unsigned long long *dst, const ResolveView &v, int ox, int oy, int rw,
+                                            int rh) {
+    const int n = rw * rh;
+    for (int i = (int)threadIdx.x; i < n; i += (int)(TILE_W * TILE_H)) {
+        const int yy = i / rw, xx = i - yy * rw;
+        const int gx = ox + xx, gy = oy + yy;
+        const bool in = gx >= 0 && gx < (int)v.w && gy >= 0 && gy < (int)v.h;
+        dst[i] = in ? v.splat[(uint32_t)gy * v.w + (uint32_t)gx] : SPLAT_EMPTY;
     }
-    return r;
+}
+
+// iterativeSearch<N>, renderer.cu:177-202. The reference always runs N
+// steps of q <- step(q). step() depends on q alone (the target pixel is
+// fixed), so once the sequence revisits a pixel it is periodic from there
+// on, and q_N can be read off the cycle without running the remaining steps:
+//   - an out-of-range candidate keeps q (the reference's `continue`), and an
+//     unchanged candidate is a fixed point: period 1, stop;
+//   - q_{i+1} == q_{i+1-p} for p = 2..4: with c_0 = q_{i+1} .. c_{p-1} = q_i,
+//     q_N = c_{(N-i-1) mod p}, taken from the last four positions.
+// This is bit-identical to the N-step loop. (At low resolution ~40% of the
+// aux searches settle into 2-cycles, which would otherwise run all N steps.)
+// Returns the cam_ff sample at the final pixel in `m`.
+__device__ __forceinline__ uint32_t pack_px(uint32_t x, uint32_t y) { return (y << 16) | x; }
+
+// One fixed-point search in flight. All searches of a pixel (fwd/bwd
+// keyframe of the main and every aux camera) advance in lock-step, so their
+// dependent gathers overlap instead of running one chain after another.
+struct Chain {
+    uint32_t h0, h1, h2, h3;  // q_i, q_{i-1}, q_{i-2}, q_{i-3} (packed)
+    float4 m;                 // cam_ff at h0
+    bool active;
+};
+constexpr uint32_t PX_NONE = 0xffffffffu;  // x, y <= 65534: never a packed pixel
+
+__device__ __forceinline__ void chain_start(Chain &c, bool run, uint32_t x, uint32_t y, const ResolveView &v) {
+    c.h0 = pack_px(x, y);
+    c.h1 = c.h2 = c.h3 = PX_NONE;
+    c.active = run;
+    c.m = run ? at(v.cam_ff, v.w, x, y) : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Step i (0-based) of n_iter: q_{i+1} = step(q_i), with the exits described above.
+__device__ __forceinline__ void chain_step(Chain &c, int i, int n_iter, float2 pcs_full, const ResolveView &v) {
+    if (!c.active) return;
+    const int2 t = cs2pixel(pcs_full.x - c.m.x, pcs_full.y - c.m.y, v.w, v.h);
+    if (!in_range(t, v.w, v.h)) {
+        c.active = false;
+        return;
+    }
+    const uint32_t nt = pack_px((uint32_t)t.x, (uint32_t)t.y);
+    if (nt == c.h0) {
+        c.active = false;
+        return;
+    }
+    const int p = nt == c.h1 ? 2 : (nt == c.h2 ? 3 : (nt == c.h3 ? 4 : 0));
+    if (p != 0) {
+        const int idx = p - 1 - (n_iter - (i + 1)) % p;  // c_r = h_{p-1-r}
+        const uint32_t q = idx == 0 ? c.h0 : (idx == 1 ? c.h1 : (idx == 2 ? c.h2 : c.h3));
+        c.h0 = q;
+        c.m = at(v.cam_ff, v.w, q & 0xFFFFu, q >> 16);
+        c.active = false;
+        return;
+    }
+    c.h3 = c.h2;
+    c.h2 = c.h1;
+    c.h1 = c.h0;
+    c.h0 = nt;
+    c.m = at(v.cam_ff, v.w, (uint32_t)t.x, (uint32_t)t.y);
 }
 
 // computeReprojError, renderer.cu:204-215
 __device__ __forceinline__ float reproj_error(uint2 q, const ResolveView &v, float2 pcs_cam, float4 m) {
-    const float2 pf = pixel2cs(q.x, q.y, v.w, v.h);
-    float ex = pf.x + m.x - pcs_cam.x;
-    float ey = pf.y + m.y - pcs_cam.y;
-    const float mn = (float)(v.w < v.h ? v.w : v.h);
-    const float sx = (float)v.w / mn, sy = (float)v.h / mn;
-    ex *= sx;
-    ey *= sy;
+    float ex = cs_coord(q.x, v.inv_w) + m.x - pcs_cam.x;
+    float ey = cs_coord(q.y, v.inv_h) + m.y - pcs_cam.y;
+    ex *= v.rs_x;
+    ey *= v.rs_y;
     return sqrtf(ex * ex + ey * ey);
 }
 
@@ -226,9 +276,9 @@ struct FrameConst {
 __device__ uchar4 second_chance(float weight, const FrameConst &fc, SplatInfo si, const ResolveView &from,
                                 const ResolveView &to, float eps2) {
     const float4 m_ft = fetch(from.full_ff, from.w, from.h, si.x, si.y);
-    const float2 spcs = pixel2cs(si.x, si.y, from.w, from.h);
     const float sz = fetch(from.z, from.w, from.h, si.x, si.y);
-    const float proj_x = spcs.x + m_ft.x, proj_y = spcs.y + m_ft.y, proj_z = sz + m_ft.z;
+    const float proj_x = cs_coord(si.x, from.inv_w) + m_ft.x, proj_y = cs_coord(si.y, from.inv_h) + m_ft.y;
+    const float proj_z = sz + m_ft.z;
     const int2 pp = cs2pixel(proj_x, proj_y, to.w, to.h);
 
     uchar4 out = color_from_shadow(fc.sidx, fc.st, fc.sfpa, fc.no_shadows, from, si.x, si.y);
@@ -245,87 +295,94 @@ __device__ uchar4 second_chance(float weight, const FrameConst &fc, SplatInfo si
     return out;
 }
 
-struct Result {
+// ReprojResult (renderer.cu:49-57) without its colour: the colour depends
+// only on the chosen candidate, so it is computed once, for the candidate
+// that is finally written (the reference computes it for the main camera and
+// for every aux camera).
+struct Cand {
     bool valid;
     bool visible_in_main;
     int priority;
     float error;
     float depth;
-    uchar4 color;
+    int dir;  // 0: fwd keyframe, 1: bwd keyframe, 2: 0.5 blend of both (ALWAYS_VALID)
 };
 
-// bidirectionalReprojection<SW, SI, ALWAYS_VALID, EPS_1, EPS_2>, renderer.cu:329-398
-template <int W, bool ALWAYS_VALID>
-__device__ Result bidir(uint32_t ix, uint32_t iy, float2 pcs_full, float2 pcs, const FrameConst &fc, int n_iter,
-                        const ResolveView &fwd, const ResolveView &bwd, float eps1, float eps2) {
-    float fwd_e = FLT_MAX;
-    bool fwd_vis = false;
-    SplatInfo fi = init_search<W>(ix, iy, fwd);
-    if (fi.depth > 0.0f && fc.alpha < 1.0f) {
-        float4 m;
-        const uint2 q = search(make_uint2(fi.x, fi.y), pcs_full, fwd, n_iter, m);
-        fi.x = q.x;
-        fi.y = q.y;
-        fi.depth = z_to_depth(fetch(fwd.z, fwd.w, fwd.h, q.x, q.y) + m.z);
-        fwd_e = reproj_error(q, fwd, pcs, m);
-        fwd_vis = m.w > 0.0f;
-    }
-    float bwd_e = FLT_MAX;
-    bool bwd_vis = false;
-    SplatInfo bi = init_search<W>(ix, iy, bwd);
-    if (bi.depth > 0.0f) {
-        float4 m;
-        const uint2 q = search(make_uint2(bi.x, bi.y), pcs_full, bwd, n_iter, m);
-        bi.x = q.x;
-        bi.y = q.y;
-        bi.depth = z_to_depth(fetch(bwd.z, bwd.w, bwd.h, q.x, q.y) + m.z);
-        bwd_e = reproj_error(q, bwd, pcs, m);
-        bwd_vis = m.w > 0.0f;
-    }
-
-    Result r{false, false, 0, 0.f, 0.f, make_uchar4(0, 0, 0, 0)};
-    if (fwd_e < eps1 || bwd_e < eps1) {
-        const bool both = fwd_e < eps1 && bwd_e < eps1;
-        const bool similar = fabsf(depth_to_z(fi.depth) - depth_to_z(bi.depth)) < eps2;
-        const bool choose_fwd = (both && !similar && fi.depth < bi.depth) || ((!both || similar) && fwd_e < bwd_e);
-        r.valid = true;
-        if (choose_fwd) {
-            r.visible_in_main = fwd_vis || (both && similar && bwd_vis);
-            r.priority = fwd.priority;
-            r.error = fwd_e;
-            r.depth = fi.depth;
-            r.color = second_chance(1.0f - fc.alpha, fc, fi, fwd, bwd, eps2);
-        } else {
-            r.visible_in_main = bwd_vis || (both && similar && fwd_vis);
-            r.priority = bwd.priority;
-            r.error = bwd_e;
-            r.depth = bi.depth;
-            r.color = second_chance(fc.alpha, fc, bi, bwd, fwd, eps2);
-        }
+// Forward/backward decision of one camera, renderer.cu:366-396.
+template <bool ALWAYS_VALID>
+__device__ __forceinline__ Cand decide(float fe, float be, float fd, float bd, bool fv, bool bv, int fprio, int bprio,
+                                       float eps1, float eps2) {
+    Cand k{false, false, 0, 0.f, 0.f, 0};
+    if (fe < eps1 || be < eps1) {
+        const bool both = fe < eps1 && be < eps1;
+        const bool similar = fabsf(depth_to_z(fd) - depth_to_z(bd)) < eps2;
+        const bool choose_fwd = (both && !similar && fd < bd) || ((!both || similar) && fe < be);
+        k.valid = true;
+        k.visible_in_main = choose_fwd ? (fv || (both && similar && bv)) : (bv || (both && similar && fv));
+        k.priority = choose_fwd ? fprio : bprio;
+        k.error = choose_fwd ? fe : be;
+        k.depth = choose_fwd ? fd : bd;
+        k.dir = choose_fwd ? 0 : 1;
     } else if (ALWAYS_VALID) {
-        r.valid = true;
-        const uchar4 cf = second_chance(1.0f - fc.alpha, fc, fi, fwd, bwd, eps2);
-        const uchar4 cb = second_chance(fc.alpha, fc, bi, bwd, fwd, eps2);
-        r.color = lerp_to_u8(cf, cb, 0.5f);
+        k.valid = true;
+        k.dir = 2;
     }
-    return r;
+    return k;
 }
 
-// bidirectionalReprojection_kernel<SW, SI, MAV>, renderer.cu:400-465
-template <int W, bool MAV>
+// Colour of a candidate of camera (fwd, bwd): renderer.cu:378-396.
+__device__ __forceinline__ uchar4 cand_colour(const Cand &k, const FrameConst &fc, SplatInfo fi, SplatInfo bi,
+                                              const ResolveView &fwd, const ResolveView &bwd, float eps2) {
+    if (k.dir == 0) return second_chance(1.0f - fc.alpha, fc, fi, fwd, bwd, eps2);
+    if (k.dir == 1) return second_chance(fc.alpha, fc, bi, bwd, fwd, eps2);
+    const uchar4 cf = second_chance(1.0f - fc.alpha, fc, fi, fwd, bwd, eps2);
+    const uchar4 cb = second_chance(fc.alpha, fc, bi, bwd, fwd, eps2);
+    return lerp_to_u8(cf, cb, 0.5f);
+}
+
+// bidirectionalReprojection_kernel<SW, SI, MAV>, renderer.cu:400-465, with
+// bidirectionalReprojection (:329-398) evaluated for the main camera and
+// NAUX aux cameras at once:
+//   1. splat-window initialisation of every (camera, keyframe) search;
+//   2. all searches in lock-step (see Chain);
+//   3. per-camera forward/backward decision (:366-396);
+//   4. aux selection and main/aux choice (:429-464);
+//   5. second chance + blend + soft shadows for the written candidate only.
+// Pixels without a valid result are not written (hole retention).
+template <int W, bool MAV, int NAUX>
 __global__ __launch_bounds__(256) void resolve_kernel(const ResolveArgs args) {
     // Read the argument block in place (constant address space) instead of
-    // through the by-value copy: the aux loop indexes it dynamically, which
-    // would otherwise spill the whole block to per-lane scratch.
+    // through a by-value copy, which the compiler may spill to scratch.
 #if defined(__HIP_DEVICE_COMPILE__)
     const KernArg<ResolveArgs> &a = *kernarg_ptr<ResolveArgs>();
 #else
     const ResolveArgs &a = args;  // host pass only parses the body
 #endif
+    constexpr int NC = 1 + NAUX;
+    constexpr int SPAN = TILE_W + 2 * W;  // staged cells per axis (TILE_W == TILE_H)
+    __shared__ unsigned long long s_splat[NC][2][SPAN * SPAN];
     const uint32_t tile = xcd_remap(blockIdx.x, a.n_tiles);
     const uint32_t ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
-    const uint32_t x = tx * TILE_W + (threadIdx.x & (TILE_W - 1));
-    const uint32_t y = ty * TILE_H + (threadIdx.x / TILE_W);
+    const uint32_t x0 = tx * TILE_W, y0 = ty * TILE_H;
+    const uint32_t x = x0 + (threadIdx.x & (TILE_W - 1));
+    const uint32_t y = y0 + (threadIdx.x / TILE_W);
+
+    // Stage every splat cell the tile's windows read: camera c covers camera
+    // pixels [x0/rx, xl/rx] (xl = last in-image column of the tile) plus a
+    // W-cell border, at most SPAN x SPAN cells.
+    int org_x[NC], org_y[NC], span_x[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const uint32_t rx = a.w / a.fwd[c].w, ry = a.h / a.fwd[c].h;
+        const uint32_t xl = min(x0 + TILE_W - 1, a.w - 1), yl = min(y0 + TILE_H - 1, a.h - 1);
+        org_x[c] = (int)(x0 / rx) - W;
+        org_y[c] = (int)(y0 / ry) - W;
+        span_x[c] = (int)(xl / rx - x0 / rx) + 1 + 2 * W;
+        const int span_y = (int)(yl / ry - y0 / ry) + 1 + 2 * W;
+        stage_splat(s_splat[c][0], a.fwd[c], org_x[c], org_y[c], span_x[c], span_y);
+        stage_splat(s_splat[c][1], a.bwd[c], org_x[c], org_y[c], span_x[c], span_y);
+    }
+    __syncthreads();
     if (x >= a.w || y >= a.h) return;
 
     FrameConst fc;
@@ -337,35 +394,116 @@ __global__ __launch_bounds__(256) void resolve_kernel(const ResolveArgs args) {
     fc.sidx = make_uint2(f2u_rd((float)a.sfpa * a.alpha + 1e-9f), f2u_ru((float)a.sfpa * a.alpha - 1e-9f));
     fc.st = (float)a.sfpa * a.alpha - (float)fc.sidx.x;
 
-    const float2 pcs = pixel2cs(x, y, a.w, a.h);
-    uchar4 *out = a.out + (size_t)y * a.out_pitch_px + x;
-    const Result main_r = bidir<W, MAV>(x, y, pcs, pcs, fc, a.max_iter, a.fwd[0], a.bwd[0], EPS_1_FULL_RES, EPS_2_FULL_RES);
-    if (main_r.valid) *out = main_r.color;
-    if (a.n_aux <= 0) return;
+    const float2 pcs = make_float2(cs_coord(x, a.fwd[0].inv_w), cs_coord(y, a.fwd[0].inv_h));
+    // per camera: pixel in the camera's own resolution (renderer.cu:432-434)
+    uint32_t cx[NC], cy[NC];
+    float2 cpcs[NC];
+    cx[0] = x;
+    cy[0] = y;
+    cpcs[0] = pcs;
+#pragma unroll
+    for (int c = 1; c < NC; c++) {
+        cx[c] = x / (a.w / a.fwd[c].w);
+        cy[c] = y / (a.h / a.fwd[c].h);
+        cpcs[c] = make_float2(cs_coord(cx[c], a.fwd[c].inv_w), cs_coord(cy[c], a.fwd[c].inv_h));
+    }
 
-    Result best{false, false, 0, 0.f, 0.f, make_uchar4(0, 0, 0, 0)};
-    for (int i = 1; i <= a.n_aux; i++) {
-        const ResolveView &fa = a.fwd[i];
-        const uint32_t ax = x / (a.w / fa.w), ay = y / (a.h / fa.h);
-        const float2 apcs = pixel2cs(ax, ay, fa.w, fa.h);
-        const Result ar = bidir<W, false>(ax, ay, pcs, apcs, fc, a.max_iter, fa, a.bwd[i], EPS_1_LOW_RES, EPS_2_LOW_RES);
-        if (!best.valid) {
-            best = ar;
-        } else if (ar.valid) {
-            const bool similar = fabsf(depth_to_z(best.depth) - depth_to_z(ar.depth)) < EPS_2_LOW_RES;
-            const bool choose_new = (!similar && ar.depth < best.depth) ||
-                                    (similar && ar.priority >= best.priority && ar.error < best.error);
-            if (choose_new) best = ar;
+    // 1. window initialisation from LDS
+    SplatInfo si[NC][2];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const int lx = (int)cx[c] - org_x[c], ly = (int)cy[c] - org_y[c];
+        si[c][0] = init_search_lds<W>(s_splat[c][0], span_x[c], lx, ly);
+        si[c][1] = init_search_lds<W>(s_splat[c][1], span_x[c], lx, ly);
+    }
+    // 2. searches in lock-step; the fwd search is skipped for alpha >= 1 (:335)
+    Chain ch[NC][2];
+    bool ran[NC][2];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        ran[c][0] = si[c][0].depth > 0.0f && fc.alpha < 1.0f;
+        ran[c][1] = si[c][1].depth > 0.0f;
+        chain_start(ch[c][0], ran[c][0], si[c][0].x, si[c][0].y, a.fwd[c]);
+        chain_start(ch[c][1], ran[c][1], si[c][1].x, si[c][1].y, a.bwd[c]);
+    }
+    for (int i = 0; i < a.max_iter; i++) {
+        bool any = false;
+#pragma unroll
+        for (int c = 0; c < NC; c++) any = any || ch[c][0].active || ch[c][1].active;
+        if (!any) break;
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            chain_step(ch[c][0], i, a.max_iter, pcs, a.fwd[c]);
+            chain_step(ch[c][1], i, a.max_iter, pcs, a.bwd[c]);
         }
     }
-    if (best.valid) {
-        bool choose_aux = !main_r.valid;
-        if (main_r.valid && !best.visible_in_main) {
-            const bool similar = fabsf(depth_to_z(best.depth) - depth_to_z(main_r.depth)) < EPS_2_LOW_RES;
-            choose_aux = !similar && best.depth < main_r.depth;
+    // depth, reprojection error and visibility at the found pixels (:338-341)
+    float err[NC][2];
+    bool vis[NC][2];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+#pragma unroll
+        for (int d = 0; d < 2; d++) {
+            const ResolveView &v = d ? a.bwd[c] : a.fwd[c];
+            err[c][d] = FLT_MAX;
+            vis[c][d] = false;
+            if (ran[c][d]) {
+                const uint32_t qx = ch[c][d].h0 & 0xFFFFu, qy = ch[c][d].h0 >> 16;
+                const float4 m = ch[c][d].m;
+                si[c][d].x = qx;
+                si[c][d].y = qy;
+                si[c][d].depth = z_to_depth(at(v.z, v.w, qx, qy) + m.z);
+                err[c][d] = reproj_error(make_uint2(qx, qy), v, cpcs[c], m);
+                vis[c][d] = m.w > 0.0f;
+            }
         }
-        if (choose_aux) *out = best.color;
     }
+    // 3.-4. per-camera decision (:366-396), folded at once into the running
+    // aux selection (:429-451) so no per-camera array outlives the loop
+    // (a dynamically indexed array would live in scratch memory).
+    Cand main_k = decide<MAV>(err[0][0], err[0][1], si[0][0].depth, si[0][1].depth, vis[0][0], vis[0][1],
+                              a.fwd[0].priority, a.bwd[0].priority, EPS_1_FULL_RES, EPS_2_FULL_RES);
+    bool aux_valid = false;
+    Cand aux_k = Cand{false, false, 0, 0.f, 0.f, 0};
+    SplatInfo aux_f = si[0][0], aux_b = si[0][1];
+    int aux_c = 0;
+#pragma unroll
+    for (int c = 1; c < NC; c++) {
+        const Cand kc = decide<false>(err[c][0], err[c][1], si[c][0].depth, si[c][1].depth, vis[c][0], vis[c][1],
+                                      a.fwd[c].priority, a.bwd[c].priority, EPS_1_LOW_RES, EPS_2_LOW_RES);
+        bool take = false;
+        if (!aux_valid) {
+            take = kc.valid;  // an invalid first result leaves "no valid best"
+        } else if (kc.valid) {
+            const bool similar = fabsf(depth_to_z(aux_k.depth) - depth_to_z(kc.depth)) < EPS_2_LOW_RES;
+            take = (!similar && kc.depth < aux_k.depth) ||
+                   (similar && kc.priority >= aux_k.priority && kc.error < aux_k.error);
+        }
+        if (take) {
+            aux_valid = true;
+            aux_k = kc;
+            aux_f = si[c][0];
+            aux_b = si[c][1];
+            aux_c = c;
+        }
+    }
+    // main vs aux (:453-464)
+    bool use_aux = false;
+    if (aux_valid) {
+        use_aux = !main_k.valid;
+        if (main_k.valid && !aux_k.visible_in_main) {
+            const bool similar = fabsf(depth_to_z(aux_k.depth) - depth_to_z(main_k.depth)) < EPS_2_LOW_RES;
+            use_aux = !similar && aux_k.depth < main_k.depth;
+        }
+    }
+    if (!use_aux && !main_k.valid) return;  // hole: keep the previous frame's pixel
+    // 5. colour of the written candidate
+    const int wc = use_aux ? aux_c : 0;
+    const Cand wk = use_aux ? aux_k : main_k;
+    const SplatInfo wf = use_aux ? aux_f : si[0][0];
+    const SplatInfo wb = use_aux ? aux_b : si[0][1];
+    const uchar4 col = cand_colour(wk, fc, wf, wb, a.fwd[wc], a.bwd[wc], wc == 0 ? EPS_2_FULL_RES : EPS_2_LOW_RES);
+    a.out[y * a.out_pitch_px + x] = col;
 }
 
 // ---------------------------------------------------------------------------
@@ -463,18 +601,27 @@ hipError_t launch_splat(const SplatArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <int W, bool MAV>
+static hipError_t launch_resolve_n(const ResolveArgs &a, hipStream_t s) {
+    const dim3 grid(a.n_tiles), block(TILE_W * TILE_H);
+    switch (a.n_aux) {
+        case 0: hipLaunchKernelGGL((resolve_kernel<W, MAV, 0>), grid, block, 0, s, a); break;
+        case 1: hipLaunchKernelGGL((resolve_kernel<W, MAV, 1>), grid, block, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((resolve_kernel<W, MAV, 2>), grid, block, 0, s, a); break;
+        case 3: hipLaunchKernelGGL((resolve_kernel<W, MAV, 3>), grid, block, 0, s, a); break;
+        default: hipLaunchKernelGGL((resolve_kernel<W, MAV, 4>), grid, block, 0, s, a); break;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_resolve(const ResolveArgs &a, int window, bool always_valid, hipStream_t s) {
     if (a.n_tiles == 0) return hipSuccess;
-    const dim3 grid(a.n_tiles), block(TILE_W * TILE_H);
-    if (window == 1 && !always_valid)
-        hipLaunchKernelGGL((resolve_kernel<1, false>), grid, block, 0, s, a);
-    else if (window == 3 && always_valid)
-        hipLaunchKernelGGL((resolve_kernel<3, true>), grid, block, 0, s, a);
-    else if (window == 1 && always_valid)
-        hipLaunchKernelGGL((resolve_kernel<1, true>), grid, block, 0, s, a);
-    else
-        hipLaunchKernelGGL((resolve_kernel<3, false>), grid, block, 0, s, a);
-    return hipGetLastError();
+    if (a.n_aux < 0 || a.n_aux > ISTI_MAX_AUX) return hipErrorInvalidValue;
+    // the reference pairs window 1 with plain validity and window 3 with
+    // main-always-valid (renderer.cu:678-689); those are the two variants built
+    if (window == 1 && !always_valid) return launch_resolve_n<1, false>(a, s);
+    if (window == 3 && always_valid) return launch_resolve_n<3, true>(a, s);
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_mean_z(const float *z, uint32_t w, uint32_t h, double *row_sums, float *mean, hipStream_t s) {

@@ -28,7 +28,7 @@ struct SplatJob {
     uint32_t w, h, n_px, block_begin;
     float weight;
     int32_t main_slot;  // which main view's VP decides visible_in_main
-    int32_t pad0, pad1;
+    double inv_w, inv_h;  // RN64(1/w), RN64(1/h) for pixel2CS
     Mat4 inv_vp_from;
     Mat4 vp_to;
 };
@@ -57,6 +57,8 @@ struct ResolveView {
     uint32_t w, h;
     int32_t priority;
     int32_t forward;
+    double inv_w, inv_h;  // RN64(1/w), RN64(1/h) for pixel2CS
+    float rs_x, rs_y;     // w / min(w, h), h / min(w, h) (renderer.cu:213)
 };
 
 struct ResolveArgs {

@@ -86,6 +86,15 @@ def test_c2_small_shadows_aux(oracle_lib, c2_small, b):
     _check(oracle_lib, cfg, fwd, bwd, b, max_iter=32)
 
 
+@pytest.mark.parametrize("iters", [1, 2, 5, 6, 7, 31, 33])
+def test_c2_small_search_cycles(oracle_lib, c2_small, iters):
+    """The aux searches fall into short cycles; the kernel reads q_N off the
+    cycle instead of stepping, which must agree with N plain steps for every
+    N (odd and even, shorter and longer than the cycle entry)."""
+    cfg, (fwd, bwd) = c2_small
+    _check(oracle_lib, cfg, fwd, bwd, 6, max_iter=iters)
+
+
 def test_c2_small_no_shadows_flag(oracle_lib, c2_small):
     cfg, (fwd, bwd) = c2_small
     _check(oracle_lib, cfg, fwd, bwd, 5, max_iter=32, no_shadows=True)
