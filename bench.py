@@ -44,8 +44,24 @@ def parse():
     p.add_argument("--mode", type=int, default=0)
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-baseline sample budget")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--traffic-file", default=str(ROOT / "profiles" / "traffic_resolve_1080p.json"))
+    p.add_argument("--traffic-file", default=str(ROOT / "profiles" / "traffic_1080p.json"))
+    p.add_argument("--copy-probe", action="store_true",
+                   help="only run a plain 256 MiB device copy 3 times (byte-counter calibration)")
     return p.parse_args()
+
+
+def copy_probe():
+    """A plain device-to-device copy of known size: under rocprofv3 --pmc its
+    FETCH_SIZE / WRITE_SIZE readings give the scale of each counter."""
+    import torch
+
+    n = 64 * 1024 * 1024
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4, "repeats": 3}))
 
 
 def algorithmic_bytes(src_px: int, out_px: int, shadows: bool) -> dict:
@@ -89,9 +105,12 @@ def cpu_baseline(fwd, bwd, infos, budget_s: float):
 
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); every run of this path "
-              "is a single-GPU workload, see DESIGN.md for how it would shard", file=sys.stderr)
+              "is a single-GPU workload (sessions, views and screen bands are the sharding units)", file=sys.stderr)
         sys.exit(2)
 
     import torch

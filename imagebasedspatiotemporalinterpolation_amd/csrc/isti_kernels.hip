@@ -141,15 +141,17 @@ __device__ __forceinline__ SplatInfo init_search_lds(const unsigned long long *t
     return best;
 }
 
-// Stages the splat cells [ox, ox+rw) x [oy, oy+rh) of view v into LDS.
+// Stages the splat cells [ox, ox+rw) x [oy, oy+rh) of view v into LDS rows
+// of constant stride SPAN (rw <= SPAN).
+template <int SPAN>
 __device__ __forceinline__ void stage_splat(unsigned long long *dst, const ResolveView &v, int ox, int oy, int rw,
                                             int rh) {
-    const int n = rw * rh;
+    const int n = SPAN * rh;
     for (int i = (int)threadIdx.x; i < n; i += (int)(TILE_W * TILE_H)) {
-        const int yy = i / rw, xx = i - yy * rw;
+        const int yy = i / SPAN, xx = i - yy * SPAN;
         const int gx = ox + xx, gy = oy + yy;
         const bool in = gx >= 0 && gx < (int)v.w && gy >= 0 && gy < (int)v.h;
-        dst[i] = in ? v.splat[(uint32_t)gy * v.w + (uint32_t)gx] : SPLAT_EMPTY;
+        if (xx < rw) dst[i] = in ? v.splat[(uint32_t)gy * v.w + (uint32_t)gx] : SPLAT_EMPTY;
     }
 }
 
@@ -176,8 +178,11 @@ struct Chain {
 };
 constexpr uint32_t PX_NONE = 0xffffffffu;  // x, y <= 65534: never a packed pixel
 
+// A chain that does not run (no splat candidate, or the skipped fwd search)
+// parks at pixel (0, 0), so the unconditional reloads in chain_step stay
+// inside the view; its result is never used.
 __device__ __forceinline__ void chain_start(Chain &c, bool run, uint32_t x, uint32_t y, const ResolveView &v) {
-    c.h0 = pack_px(x, y);
+    c.h0 = run ? pack_px(x, y) : 0u;
     c.h1 = c.h2 = c.h3 = PX_NONE;
     c.active = run;
     c.m = run ? at(v.cam_ff, v.w, x, y) : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -379,8 +384,8 @@ __global__ __launch_bounds__(256) void resolve_kernel(const ResolveArgs args) {
         org_y[c] = (int)(y0 / ry) - W;
         span_x[c] = (int)(xl / rx - x0 / rx) + 1 + 2 * W;
         const int span_y = (int)(yl / ry - y0 / ry) + 1 + 2 * W;
-        stage_splat(s_splat[c][0], a.fwd[c], org_x[c], org_y[c], span_x[c], span_y);
-        stage_splat(s_splat[c][1], a.bwd[c], org_x[c], org_y[c], span_x[c], span_y);
+        stage_splat<SPAN>(s_splat[c][0], a.fwd[c], org_x[c], org_y[c], span_x[c], span_y);
+        stage_splat<SPAN>(s_splat[c][1], a.bwd[c], org_x[c], org_y[c], span_x[c], span_y);
     }
     __syncthreads();
     if (x >= a.w || y >= a.h) return;
@@ -413,8 +418,8 @@ __global__ __launch_bounds__(256) void resolve_kernel(const ResolveArgs args) {
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         const int lx = (int)cx[c] - org_x[c], ly = (int)cy[c] - org_y[c];
-        si[c][0] = init_search_lds<W>(s_splat[c][0], span_x[c], lx, ly);
-        si[c][1] = init_search_lds<W>(s_splat[c][1], span_x[c], lx, ly);
+        si[c][0] = init_search_lds<W>(s_splat[c][0], SPAN, lx, ly);
+        si[c][1] = init_search_lds<W>(s_splat[c][1], SPAN, lx, ly);
     }
     // 2. searches in lock-step; the fwd search is skipped for alpha >= 1 (:335)
     Chain ch[NC][2];
