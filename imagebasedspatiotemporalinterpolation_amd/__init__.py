@@ -13,6 +13,7 @@ Python entry points:
 from .renderer import (  # noqa: F401
     AdditionalRenderInfo,
     AuxCamData,
+    Batch,
     FrameData,
     KeyframeView,
     RenderInfo,

@@ -35,6 +35,11 @@ EXPORTED_SYMBOLS = (
     "isti_stage_totals",
     "isti_last_error",
     "isti_version",
+    "isti_batch_create",
+    "isti_batch_destroy",
+    "isti_batch_set_graph",
+    "isti_render_batch",
+    "isti_batch_last_error",
 )
 
 
@@ -134,6 +139,17 @@ def lib() -> ctypes.CDLL:
     L.isti_last_error.restype = c_char_p
     L.isti_version.argtypes = []
     L.isti_version.restype = c_char_p
+    L.isti_batch_create.argtypes = [c_int32, c_int32, POINTER(c_void_p)]
+    L.isti_batch_create.restype = c_int32
+    L.isti_batch_destroy.argtypes = [c_void_p]
+    L.isti_batch_destroy.restype = c_int32
+    L.isti_batch_set_graph.argtypes = [c_void_p, c_int32]
+    L.isti_batch_set_graph.restype = c_int32
+    L.isti_render_batch.argtypes = [c_void_p, POINTER(c_void_p), POINTER(RenderInfoC), POINTER(c_void_p),
+                                    POINTER(c_size_t), c_int32, c_void_p]
+    L.isti_render_batch.restype = c_int32
+    L.isti_batch_last_error.argtypes = [c_void_p]
+    L.isti_batch_last_error.restype = c_char_p
     _lib = L
     return L
 

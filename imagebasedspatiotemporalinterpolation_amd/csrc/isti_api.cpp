@@ -10,10 +10,14 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <new>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "../../include/isti.h"
 #include "isti_kernels.hpp"
@@ -234,9 +238,331 @@ isti_status render_timewarp(isti_renderer *r, const isti_render_info *info, uint
     return ISTI_OK;
 }
 
+// Everything one frame (modes 0-3) launches, before block offsets are
+// assigned: splat jobs ordered fwd main, bwd main, then per aux camera
+// curr / next with the reference's weights and matrices
+// (src/renderer.cu:643-649), and the resolve descriptor (:652-693).
+struct FramePlan {
+    SplatJob jobs[MAX_SPLAT_JOBS];
+    int nj = 0;
+    Mat4 vp_cam;
+    Mat4 vp_main[2];
+    int32_t init_linear = 0;
+    ResolveArgs ra;
+    int window = 1;
+    bool always_valid = false;
+};
+
+isti_status plan_frame(isti_renderer *r, const isti_render_info *info, uint8_t *out, size_t out_pitch,
+                       FramePlan &P) {
+    const int nv = 1 + r->n_aux;
+    for (int sl = 0; sl < 2; sl++)
+        for (int v = 0; v < nv; v++) {
+            const isti_status st = ensure_scratch(r, r->scratch[sl][v], r->views[sl][v].width, r->views[sl][v].height);
+            if (st != ISTI_OK) return st;
+        }
+    std::memset(P.jobs, 0, sizeof(P.jobs));
+    P.vp_cam = to_mat(info->viewproj);
+    P.vp_main[0] = to_mat(r->views[0][0].viewproj);
+    P.vp_main[1] = to_mat(r->views[1][0].viewproj);
+    P.init_linear = info->mode == ISTI_MODE_BACKWARD_BIDIRECTIONAL ? 1 : 0;
+    P.nj = 0;
+    for (int v = 0; v < nv; v++) {
+        for (int sl = 0; sl < 2; sl++) {
+            const isti_view_desc &d = r->views[sl][v];
+            const isti_view_desc &other = r->views[1 - sl][v];
+            Scratch &sc = r->scratch[sl][v];
+            SplatJob &J = P.jobs[P.nj++];
+            const bool hm = has_motion(d);
+            J.z = d.z;
+            J.mx = hm ? d.motion_x : nullptr;
+            J.my = hm ? d.motion_y : nullptr;
+            J.mz = hm ? d.motion_z : nullptr;
+            J.full_ff = sc.full_ff;
+            J.cam_ff = sc.cam_ff;
+            J.splat = sc.splat;
+            J.w = d.width;
+            J.h = d.height;
+            J.n_px = d.width * d.height;
+            J.weight = sl == 0 ? info->alpha : 1.0f - info->alpha;
+            J.main_slot = sl;
+            J.inv_w = 1.0 / (double)d.width;
+            J.inv_h = 1.0 / (double)d.height;
+            J.inv_vp_from = to_mat(d.inv_viewproj);
+            J.vp_to = to_mat(other.viewproj);
+        }
+    }
+    ResolveArgs &ra = P.ra;
+    std::memset(&ra, 0, sizeof(ra));
+    for (int v = 0; v < nv; v++) {
+        const int prio_f = v == 0 ? 0 : r->views[0][v].priority;  // main priority is 0 (renderer.cu:654)
+        const int prio_b = v == 0 ? 0 : r->views[1][v].priority;
+        ra.fwd[v] = resolve_view(r->views[0][v], r->scratch[0][v], true, prio_f);
+        ra.bwd[v] = resolve_view(r->views[1][v], r->scratch[1][v], false, prio_b);
+    }
+    ra.out = reinterpret_cast<uchar4 *>(out);
+    ra.out_pitch_px = (uint32_t)(out_pitch / 4);
+    ra.w = info->width;
+    ra.h = info->height;
+    ra.alpha = info->alpha;
+    ra.fpa = info->frames_per_anchor;
+    ra.sfpa = info->shadow_frames_per_anchor;
+    ra.no_shadows = info->no_shadows;
+    ra.n_aux = r->n_aux;
+    int iters = 3;
+    P.window = 1;
+    P.always_valid = false;
+    switch (info->mode) {  // src/renderer.cu:676-693
+        case ISTI_MODE_OURS:
+        case ISTI_MODE_BACKWARD_UNIDIRECTIONAL: iters = 3; break;
+        case ISTI_MODE_SPLATTING: iters = 0; break;
+        case ISTI_MODE_BACKWARD_BIDIRECTIONAL: P.window = 3; iters = 5; P.always_valid = true; break;
+        default: break;
+    }
+    ra.max_iter = info->max_search_iterations >= 0 ? info->max_search_iterations : iters;
+    ra.tiles_x = (ra.w + TILE_W - 1) / TILE_W;
+    ra.n_tiles = ra.tiles_x * ((ra.h + TILE_H - 1) / TILE_H);
+    return ISTI_OK;
+}
+
+constexpr size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace
+
+// Double-buffered device work tables for batched renders (see isti.h).
+struct isti_batch {
+    int device = 0;
+    int cap_req = 0, cap_jobs = 0;
+    size_t bytes = 0;
+    uint8_t *host[2] = {nullptr, nullptr};  // pinned staging
+    uint8_t *dev[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    bool in_use[2] = {false, false};
+    int slot = 0;
+    bool graph = false;
+    uint32_t grid_light = 0;
+    uint32_t grid_resolve[N_VARIANTS] = {};
+    std::map<std::pair<int, uint32_t>, hipGraphExec_t> graphs;  // (slot, variant mask)
+    hipStream_t capture = nullptr;
+    std::vector<FramePlan> plans;
+    std::string err;
+};
+
+namespace {
+
+isti_status bfail(isti_batch *b, isti_status st, const std::string &msg) {
+    if (b) b->err = msg;
+    return st;
+}
+
+// Launches the stage sequence of one table slot (or records it into a capture).
+hipError_t launch_batch_stages(isti_batch *b, const BatchTable *t, uint32_t mask, hipStream_t s) {
+    hipError_t e = launch_clear_batch(t, b->grid_light, s);
+    if (e == hipSuccess) e = launch_splat_batch(t, b->grid_light, s);
+    for (int v = 0; v < N_VARIANTS && e == hipSuccess; v++)
+        if (mask & (1u << v)) e = launch_resolve_batch(t, v, b->grid_resolve[v], s);
+    return e;
+}
+
 }  // namespace
 
 extern "C" {
+
+isti_status isti_batch_create(int32_t device, int32_t max_requests, isti_batch **out) {
+    if (!out || max_requests <= 0 || max_requests > 4096) {
+        g_create_error = "out must be non-null and max_requests in 1..4096";
+        return ISTI_ERR_INVALID_ARG;
+    }
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+        g_create_error = "no such HIP device";
+        return ISTI_ERR_DEVICE;
+    }
+    isti_batch *b = new (std::nothrow) isti_batch();
+    if (!b) return ISTI_ERR_OOM;
+    b->device = device;
+    b->cap_req = max_requests;
+    b->cap_jobs = max_requests * MAX_SPLAT_JOBS;
+    b->bytes = align_up(sizeof(BatchTable), 256) + align_up(sizeof(BatchRequest) * b->cap_req, 256) +
+               align_up(sizeof(SplatJob) * b->cap_jobs, 256) + 3 * align_up(sizeof(uint32_t) * (b->cap_jobs + 1), 256);
+    hipError_t e = hipSetDevice(device);
+    for (int i = 0; i < 2 && e == hipSuccess; i++) {
+        e = hipHostMalloc(&b->host[i], b->bytes, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipMalloc(&b->dev[i], b->bytes);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done[i], hipEventDisableTiming);
+    }
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking);
+    int cus = 0;
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    if (e != hipSuccess) {
+        g_create_error = std::string("batch setup: ") + hipGetErrorString(e);
+        isti_batch_destroy(b);
+        return e == hipErrorOutOfMemory ? ISTI_ERR_OOM : ISTI_ERR_DEVICE;
+    }
+    const uint32_t ncu = (uint32_t)(cus > 0 ? cus : 1);
+    b->grid_light = std::max<uint32_t>(8, ncu * 8 / 8 * 8);
+    for (int v = 0; v < N_VARIANTS; v++)
+        b->grid_resolve[v] = std::max<uint32_t>(8, ncu * (uint32_t)resolve_batch_blocks_per_cu(v) / 8 * 8);
+    b->plans.resize(b->cap_req);
+    *out = b;
+    return ISTI_OK;
+}
+
+isti_status isti_batch_destroy(isti_batch *b) {
+    if (!b) return ISTI_ERR_INVALID_ARG;
+    (void)hipSetDevice(b->device);
+    for (int i = 0; i < 2; i++) {
+        if (b->done[i]) (void)hipEventSynchronize(b->done[i]);
+    }
+    for (auto &g : b->graphs) (void)hipGraphExecDestroy(g.second);
+    for (int i = 0; i < 2; i++) {
+        if (b->host[i]) (void)hipHostFree(b->host[i]);
+        if (b->dev[i]) (void)hipFree(b->dev[i]);
+        if (b->done[i]) (void)hipEventDestroy(b->done[i]);
+    }
+    if (b->capture) (void)hipStreamDestroy(b->capture);
+    delete b;
+    return ISTI_OK;
+}
+
+isti_status isti_batch_set_graph(isti_batch *b, int32_t enable) {
+    if (!b) return ISTI_ERR_INVALID_ARG;
+    b->graph = enable != 0;
+    return ISTI_OK;
+}
+
+isti_status isti_render_batch(isti_batch *b, isti_renderer *const *rs, const isti_render_info *infos,
+                              uint8_t *const *outs, const size_t *pitches, int32_t n, void *stream) {
+    if (!b) return ISTI_ERR_INVALID_ARG;
+    if (n == 0) return ISTI_OK;
+    if (n < 0 || n > b->cap_req || !rs || !infos || !outs || !pitches)
+        return bfail(b, ISTI_ERR_INVALID_ARG, "need 0 <= n <= max_requests and non-null arrays");
+    hipError_t e = hipSetDevice(b->device);
+    if (e != hipSuccess) return bfail(b, ISTI_ERR_DEVICE, hipGetErrorString(e));
+    // validate and plan every request
+    std::vector<int> var(n);
+    for (int i = 0; i < n; i++) {
+        isti_renderer *r = rs[i];
+        if (!r || !outs[i]) return bfail(b, ISTI_ERR_INVALID_ARG, "null renderer or output in request " + std::to_string(i));
+        if (r->device != b->device) return bfail(b, ISTI_ERR_INVALID_ARG, "renderer on another device");
+        for (int k = 0; k < i; k++)
+            if (rs[k] == r) return bfail(b, ISTI_ERR_INVALID_ARG, "a renderer appears twice in one batch");
+        if (infos[i].mode == ISTI_MODE_TIMEWARP) return bfail(b, ISTI_ERR_INVALID_ARG, "timewarp is not batched");
+        isti_status st = validate(r, &infos[i], pitches[i]);
+        if (st == ISTI_OK) st = plan_frame(r, &infos[i], outs[i], pitches[i], b->plans[i]);
+        if (st != ISTI_OK) return bfail(b, st, "request " + std::to_string(i) + ": " + r->err);
+        var[i] = resolve_variant(b->plans[i].always_valid, r->n_aux);
+    }
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return var[x] < var[y]; });
+
+    // the staging buffer of this slot may still feed an earlier batch's copy
+    const int sl = b->slot;
+    if (b->in_use[sl]) {
+        e = hipEventSynchronize(b->done[sl]);
+        if (e != hipSuccess) return bfail(b, ISTI_ERR_DEVICE, hipGetErrorString(e));
+    }
+    uint8_t *h = b->host[sl];
+    uint8_t *d = b->dev[sl];
+    int nj = 0;
+    for (int i = 0; i < n; i++) nj += b->plans[i].nj;
+    size_t off = align_up(sizeof(BatchTable), 256);
+    const size_t off_req = off;
+    off = align_up(off + sizeof(BatchRequest) * n, 256);
+    const size_t off_jobs = off;
+    off = align_up(off + sizeof(SplatJob) * nj, 256);
+    const size_t off_clear = off;
+    off = align_up(off + sizeof(uint32_t) * (nj + 1), 256);
+    const size_t off_splat = off;
+    off = align_up(off + sizeof(uint32_t) * (nj + 1), 256);
+    const size_t off_tile = off;
+    const size_t used = off + sizeof(uint32_t) * (n + 1);
+
+    BatchTable *T = reinterpret_cast<BatchTable *>(h);
+    std::memset(T, 0, sizeof(BatchTable));
+    BatchRequest *req = reinterpret_cast<BatchRequest *>(h + off_req);
+    SplatJob *jobs = reinterpret_cast<SplatJob *>(h + off_jobs);
+    uint32_t *clear_begin = reinterpret_cast<uint32_t *>(h + off_clear);
+    uint32_t *splat_begin = reinterpret_cast<uint32_t *>(h + off_splat);
+    uint32_t *tile_begin = reinterpret_cast<uint32_t *>(h + off_tile);
+    uint32_t mask = 0, cb = 0, sb = 0, tb = 0;
+    int j = 0;
+    for (int q = 0; q < n; q++) {
+        const FramePlan &P = b->plans[order[q]];
+        BatchRequest &R = req[q];
+        std::memset(&R, 0, sizeof(R));
+        R.args = P.ra;
+        R.vp_cam = P.vp_cam;
+        R.vp_main[0] = P.vp_main[0];
+        R.vp_main[1] = P.vp_main[1];
+        R.init_linear = P.init_linear;
+        tile_begin[q] = tb;
+        tb += P.ra.n_tiles;
+        mask |= 1u << var[order[q]];
+        for (int k = 0; k < P.nj; k++, j++) {
+            jobs[j] = P.jobs[k];
+            jobs[j].req = q;
+            clear_begin[j] = cb;
+            cb += (P.jobs[k].n_px + 511u) / 512u;
+            splat_begin[j] = sb;
+            sb += (P.jobs[k].n_px + SPLAT_BLOCK - 1) / SPLAT_BLOCK;
+        }
+    }
+    clear_begin[nj] = cb;
+    splat_begin[nj] = sb;
+    tile_begin[n] = tb;
+    T->n_req = (uint32_t)n;
+    T->n_jobs = (uint32_t)nj;
+    T->clear_blocks = cb;
+    T->splat_blocks = sb;
+    for (int v = 0, q = 0; v <= N_VARIANTS; v++) {
+        while (q < n && var[order[q]] < v) q++;
+        T->var_req_begin[v] = (uint32_t)q;
+    }
+    T->req = reinterpret_cast<const BatchRequest *>(d + off_req);
+    T->jobs = reinterpret_cast<const SplatJob *>(d + off_jobs);
+    T->clear_begin = reinterpret_cast<const uint32_t *>(d + off_clear);
+    T->splat_begin = reinterpret_cast<const uint32_t *>(d + off_splat);
+    T->tile_begin = reinterpret_cast<const uint32_t *>(d + off_tile);
+
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    e = hipMemcpyAsync(d, h, used, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return bfail(b, ISTI_ERR_DEVICE, std::string("table copy: ") + hipGetErrorString(e));
+    const BatchTable *td = reinterpret_cast<const BatchTable *>(d);
+    if (b->graph) {
+        const auto key = std::make_pair(sl, mask);
+        auto it = b->graphs.find(key);
+        if (it == b->graphs.end()) {
+            hipGraph_t g = nullptr;
+            hipGraphExec_t ex = nullptr;
+            e = hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) {
+                const hipError_t le = launch_batch_stages(b, td, mask, b->capture);
+                e = hipStreamEndCapture(b->capture, &g);
+                if (e == hipSuccess) e = le;
+            }
+            if (e == hipSuccess) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            if (g) (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return bfail(b, ISTI_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+            it = b->graphs.emplace(key, ex).first;
+        }
+        e = hipGraphLaunch(it->second, s);
+    } else {
+        e = launch_batch_stages(b, td, mask, s);
+    }
+    if (e != hipSuccess) return bfail(b, ISTI_ERR_DEVICE, std::string("batch launch: ") + hipGetErrorString(e));
+    e = hipEventRecord(b->done[sl], s);
+    if (e != hipSuccess) return bfail(b, ISTI_ERR_DEVICE, hipGetErrorString(e));
+    b->in_use[sl] = true;
+    b->slot = 1 - sl;
+    return ISTI_OK;
+}
+
+const char *isti_batch_last_error(const isti_batch *b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+
 
 isti_status isti_create(const isti_config *config, isti_renderer **out) {
     if (!config || !out) {
@@ -320,94 +646,33 @@ isti_status isti_render(isti_renderer *r, const isti_render_info *info, uint8_t 
 
     if (info->mode == ISTI_MODE_TIMEWARP) return render_timewarp(r, info, out, out_pitch, s);
 
-    const int nv = 1 + r->n_aux;
-    for (int sl = 0; sl < 2; sl++)
-        for (int v = 0; v < nv; v++) {
-            st = ensure_scratch(r, r->scratch[sl][v], r->views[sl][v].width, r->views[sl][v].height);
-            if (st != ISTI_OK) return st;
-        }
+    FramePlan P;
+    st = plan_frame(r, info, out, out_pitch, P);
+    if (st != ISTI_OK) return st;
 
-    // splat clear + reproject/splat: jobs ordered fwd main, bwd main, then
-    // per aux camera curr / next, with the reference's weights and matrices
-    // (src/renderer.cu:643-649)
     ClearArgs ca;
     std::memset(&ca, 0, sizeof(ca));
     SplatArgs sa;
     std::memset(&sa, 0, sizeof(sa));
-    sa.vp_cam = to_mat(info->viewproj);
-    sa.vp_main[0] = to_mat(r->views[0][0].viewproj);
-    sa.vp_main[1] = to_mat(r->views[1][0].viewproj);
-    sa.init_linear = info->mode == ISTI_MODE_BACKWARD_BIDIRECTIONAL ? 1 : 0;
+    sa.vp_cam = P.vp_cam;
+    sa.vp_main[0] = P.vp_main[0];
+    sa.vp_main[1] = P.vp_main[1];
+    sa.init_linear = P.init_linear;
     uint32_t clear_blocks = 0, splat_blocks = 0;
-    int nj = 0;
-    for (int v = 0; v < nv; v++) {
-        for (int sl = 0; sl < 2; sl++) {
-            const isti_view_desc &d = r->views[sl][v];
-            const isti_view_desc &other = r->views[1 - sl][v];
-            Scratch &sc = r->scratch[sl][v];
-            const uint32_t n = d.width * d.height;
-            ca.buf[nj] = sc.splat;
-            ca.n[nj] = n;
-            ca.block_begin[nj] = clear_blocks;
-            clear_blocks += (n + 511u) / 512u;
-
-            SplatJob &J = sa.job[nj];
-            const bool hm = has_motion(d);
-            J.z = d.z;
-            J.mx = hm ? d.motion_x : nullptr;
-            J.my = hm ? d.motion_y : nullptr;
-            J.mz = hm ? d.motion_z : nullptr;
-            J.full_ff = sc.full_ff;
-            J.cam_ff = sc.cam_ff;
-            J.splat = sc.splat;
-            J.w = d.width;
-            J.h = d.height;
-            J.n_px = n;
-            J.block_begin = splat_blocks;
-            splat_blocks += (n + SPLAT_BLOCK - 1) / SPLAT_BLOCK;
-            J.weight = sl == 0 ? info->alpha : 1.0f - info->alpha;
-            J.main_slot = sl;
-            J.inv_w = 1.0 / (double)d.width;
-            J.inv_h = 1.0 / (double)d.height;
-            J.inv_vp_from = to_mat(d.inv_viewproj);
-            J.vp_to = to_mat(other.viewproj);
-            nj++;
-        }
+    for (int j = 0; j < P.nj; j++) {
+        SplatJob &J = sa.job[j];
+        J = P.jobs[j];
+        ca.buf[j] = J.splat;
+        ca.n[j] = J.n_px;
+        ca.block_begin[j] = clear_blocks;
+        clear_blocks += (J.n_px + 511u) / 512u;
+        J.block_begin = splat_blocks;
+        splat_blocks += (J.n_px + SPLAT_BLOCK - 1) / SPLAT_BLOCK;
     }
-    ca.n_bufs = nj;
+    ca.n_bufs = P.nj;
     ca.n_blocks = clear_blocks;
-    sa.n_jobs = nj;
+    sa.n_jobs = P.nj;
     sa.n_blocks = splat_blocks;
-
-    ResolveArgs ra;
-    std::memset(&ra, 0, sizeof(ra));
-    for (int v = 0; v < nv; v++) {
-        const int prio_f = v == 0 ? 0 : r->views[0][v].priority;  // main priority is 0 (renderer.cu:654)
-        const int prio_b = v == 0 ? 0 : r->views[1][v].priority;
-        ra.fwd[v] = resolve_view(r->views[0][v], r->scratch[0][v], true, prio_f);
-        ra.bwd[v] = resolve_view(r->views[1][v], r->scratch[1][v], false, prio_b);
-    }
-    ra.out = reinterpret_cast<uchar4 *>(out);
-    ra.out_pitch_px = (uint32_t)(out_pitch / 4);
-    ra.w = info->width;
-    ra.h = info->height;
-    ra.alpha = info->alpha;
-    ra.fpa = info->frames_per_anchor;
-    ra.sfpa = info->shadow_frames_per_anchor;
-    ra.no_shadows = info->no_shadows;
-    ra.n_aux = r->n_aux;
-    int window = 1, iters = 3;
-    bool always_valid = false;
-    switch (info->mode) {  // src/renderer.cu:676-693
-        case ISTI_MODE_OURS:
-        case ISTI_MODE_BACKWARD_UNIDIRECTIONAL: window = 1; iters = 3; break;
-        case ISTI_MODE_SPLATTING: window = 1; iters = 0; break;
-        case ISTI_MODE_BACKWARD_BIDIRECTIONAL: window = 3; iters = 5; always_valid = true; break;
-        default: break;
-    }
-    ra.max_iter = info->max_search_iterations >= 0 ? info->max_search_iterations : iters;
-    ra.tiles_x = (ra.w + TILE_W - 1) / TILE_W;
-    ra.n_tiles = ra.tiles_x * ((ra.h + TILE_H - 1) / TILE_H);
 
     if ((e = mark(r, 0, s)) != hipSuccess) return dev_fail(r, e, "event");
     e = launch_clear(ca, s);
@@ -416,7 +681,7 @@ isti_status isti_render(isti_renderer *r, const isti_render_info *info, uint8_t 
     e = launch_splat(sa, s);
     if (e != hipSuccess) return dev_fail(r, e, "splat launch");
     if ((e = mark(r, 2, s)) != hipSuccess) return dev_fail(r, e, "event");
-    e = launch_resolve(ra, window, always_valid, s);
+    e = launch_resolve(P.ra, P.window, P.always_valid, s);
     if (e != hipSuccess) return dev_fail(r, e, "resolve launch");
     if ((e = mark(r, 3, s)) != hipSuccess) return dev_fail(r, e, "event");
     return ISTI_OK;

@@ -40,20 +40,21 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
 // ---------------------------------------------------------------------------
 // Splat clear: 16-byte stores over every buffer of the frame.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void clear_splat_kernel(const ClearArgs a) {
-    const uint32_t b = blockIdx.x;
-    int j = 0;
-    for (int i = 1; i < a.n_bufs; i++)
-        if (b >= a.block_begin[i]) j = i;
-    unsigned long long *buf = a.buf[j];
-    const uint32_t n = a.n[j];
-    const uint32_t pair = (b - a.block_begin[j]) * 256u + threadIdx.x;  // 2 cells per thread
-    const uint32_t i0 = pair * 2u;
+__device__ __forceinline__ void clear_pairs(unsigned long long *buf, uint32_t n, uint32_t pair) {
+    const uint32_t i0 = pair * 2u;  // 2 cells per thread
     if (i0 + 1u < n) {
         reinterpret_cast<ulonglong2 *>(buf)[pair] = make_ulonglong2(SPLAT_EMPTY, SPLAT_EMPTY);
     } else if (i0 < n) {
         buf[i0] = SPLAT_EMPTY;
     }
+}
+
+__global__ __launch_bounds__(256) void clear_splat_kernel(const ClearArgs a) {
+    const uint32_t b = blockIdx.x;
+    int j = 0;
+    for (int i = 1; i < a.n_bufs; i++)
+        if (b >= a.block_begin[i]) j = i;
+    clear_pairs(a.buf[j], a.n[j], (b - a.block_begin[j]) * 256u + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -62,13 +63,9 @@ __global__ __launch_bounds__(256) void clear_splat_kernel(const ClearArgs a) {
 // float4 flow stores are fully coalesced; the u64 atomicMin is the only
 // scattered access.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void reproject_splat_kernel(const SplatArgs a) {
-    const uint32_t b = xcd_remap(blockIdx.x, a.n_blocks);
-    int j = 0;
-    for (int i = 1; i < a.n_jobs; i++)
-        if (b >= a.job[i].block_begin) j = i;
-    const SplatJob &J = a.job[j];
-    const uint32_t p = (b - J.block_begin) * SPLAT_BLOCK + threadIdx.x;
+template <typename J_t, typename M_t>
+__device__ __forceinline__ void splat_pixel(const J_t &J, const M_t &vp_cam, const M_t &vp_main, bool init_linear,
+                                            uint32_t p) {
     if (p >= J.n_px) return;
     const uint32_t y = p / J.w;
     const uint32_t x = p - y * J.w;
@@ -82,7 +79,7 @@ __global__ __launch_bounds__(256) void reproject_splat_kernel(const SplatArgs a)
     const float4 cs_moved = make_float4(cs_from.x + om.x, cs_from.y + om.y, cs_from.z + om.z, cs_from.w + 0.0f);
     const float4 ws_to = project(J.inv_vp_from, cs_moved);
 
-    const float4 cs_main = project(a.vp_main[J.main_slot], ws_from);
+    const float4 cs_main = project(vp_main, ws_from);
     const float lim = 1.0f - 1e-3f;
     const bool visible = cs_main.x > -lim && cs_main.x < lim && cs_main.y > -lim && cs_main.y < lim &&
                          cs_main.z > -lim && cs_main.z < lim;
@@ -97,8 +94,8 @@ __global__ __launch_bounds__(256) void reproject_splat_kernel(const SplatArgs a)
 
     float depth = z_to_depth(cs_from.z);
     uint32_t tx = x, ty = y;
-    if (!a.init_linear) {
-        const float4 cs_cam = project(a.vp_cam, ws_scaled);
+    if (!init_linear) {
+        const float4 cs_cam = project(vp_cam, ws_scaled);
         J.cam_ff[p] = make_float4(cs_cam.x - cs_from.x, cs_cam.y - cs_from.y, cs_cam.z - cs_from.z, vis);
         const float depth_cam = z_to_depth(cs_cam.z);
         const int2 pc = cs2pixel(cs_cam.x, cs_cam.y, J.w, J.h);
@@ -110,6 +107,15 @@ __global__ __launch_bounds__(256) void reproject_splat_kernel(const SplatArgs a)
         J.cam_ff[p] = make_float4(fx * wt, fy * wt, fz * wt, vis);
     }
     atomicMin(&J.splat[ty * J.w + tx], encode_key(depth, x, y));
+}
+
+__global__ __launch_bounds__(256) void reproject_splat_kernel(const SplatArgs a) {
+    const uint32_t b = xcd_remap(blockIdx.x, a.n_blocks);
+    int j = 0;
+    for (int i = 1; i < a.n_jobs; i++)
+        if (b >= a.job[i].block_begin) j = i;
+    const SplatJob &J = a.job[j];
+    splat_pixel(J, a.vp_cam, a.vp_main[J.main_slot], a.init_linear != 0, (b - J.block_begin) * SPLAT_BLOCK + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -354,19 +360,20 @@ __device__ __forceinline__ uchar4 cand_colour(const Cand &k, const FrameConst &f
 //   4. aux selection and main/aux choice (:429-464);
 //   5. second chance + blend + soft shadows for the written candidate only.
 // Pixels without a valid result are not written (hole retention).
-template <int W, bool MAV, int NAUX>
-__global__ __launch_bounds__(256) void resolve_kernel(const ResolveArgs args) {
-    // Read the argument block in place (constant address space) instead of
-    // through a by-value copy, which the compiler may spill to scratch.
-#if defined(__HIP_DEVICE_COMPILE__)
-    const KernArg<ResolveArgs> &a = *kernarg_ptr<ResolveArgs>();
-#else
-    const ResolveArgs &a = args;  // host pass only parses the body
-#endif
+template <int W, int NAUX>
+struct ResolveLds {
+    static constexpr int NC = 1 + NAUX;
+    static constexpr int SPAN = TILE_W + 2 * W;  // staged cells per axis (TILE_W == TILE_H)
+    unsigned long long splat[NC][2][SPAN * SPAN];
+};
+
+// One 16x16 tile of one render. Every thread of the block must call it (the
+// LDS staging ends in a barrier); pixels outside the image return early.
+template <int W, bool MAV, int NAUX, typename A>
+__device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveLds<W, NAUX> &lds) {
     constexpr int NC = 1 + NAUX;
-    constexpr int SPAN = TILE_W + 2 * W;  // staged cells per axis (TILE_W == TILE_H)
-    __shared__ unsigned long long s_splat[NC][2][SPAN * SPAN];
-    const uint32_t tile = xcd_remap(blockIdx.x, a.n_tiles);
+    constexpr int SPAN = TILE_W + 2 * W;
+    auto &s_splat = lds.splat;
     const uint32_t ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
     const uint32_t x0 = tx * TILE_W, y0 = ty * TILE_H;
     const uint32_t x = x0 + (threadIdx.x & (TILE_W - 1));
@@ -511,6 +518,106 @@ __global__ __launch_bounds__(256) void resolve_kernel(const ResolveArgs args) {
     a.out[y * a.out_pitch_px + x] = col;
 }
 
+template <int W, bool MAV, int NAUX>
+__global__ __launch_bounds__(256) void resolve_kernel(const ResolveArgs args) {
+    // Read the argument block in place (constant address space) instead of
+    // through a by-value copy, which the compiler may spill to scratch.
+#if defined(__HIP_DEVICE_COMPILE__)
+    const KernArg<ResolveArgs> &a = *kernarg_ptr<ResolveArgs>();
+#else
+    const ResolveArgs &a = args;  // host pass only parses the body
+#endif
+    __shared__ ResolveLds<W, NAUX> lds;
+    resolve_tile<W, MAV, NAUX>(a, xcd_remap(blockIdx.x, a.n_tiles), lds);
+}
+
+// ---------------------------------------------------------------------------
+// Batched, persistent variants: a fixed grid walks a device-resident work
+// table (many renders of uneven size, one launch per stage). The grid and
+// the arguments never change between batches, so the launch sequence can be
+// replayed as a hipGraph; only the table contents differ.
+// Work item k of T is owned by XCD x = (its block id) % 8, which walks a
+// contiguous range of items, so neighbouring tiles share an L2.
+// ---------------------------------------------------------------------------
+struct XcdRange {
+    uint32_t begin, end, stride, first;
+};
+__device__ __forceinline__ XcdRange xcd_range(uint32_t total) {
+    constexpr uint32_t NX = 8;
+    const uint32_t x = blockIdx.x % NX, j = blockIdx.x / NX, per = gridDim.x / NX;
+    const uint32_t q = total / NX, r = total % NX;
+    const uint32_t b = x * q + (x < r ? x : r);
+    return XcdRange{b, b + q + (x < r ? 1u : 0u), per, b + j};
+}
+
+// largest i in [0, n) with begin[i] <= k (begin[0] == 0, ascending)
+__device__ __forceinline__ uint32_t find_owner(const uint32_t *begin, uint32_t n, uint32_t k) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (begin[mid] <= k) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// The work table is read-only while a batch runs: reading it through the
+// constant address space lets the compiler keep descriptors in scalar
+// registers, as it does for kernel arguments.
+template <typename T>
+__device__ __forceinline__ KernArg<T> *as_const(const T *p) {
+    return (KernArg<T> *)p;
+}
+#if defined(__HIP_DEVICE_COMPILE__)
+#define ISTI_CONST_TABLE(t, tg) const KernArg<BatchTable> *t = as_const(tg)
+#else
+#define ISTI_CONST_TABLE(t, tg) const BatchTable *t = tg  // host pass only parses the body
+#endif
+
+__global__ __launch_bounds__(256) void clear_batch_kernel(const BatchTable *__restrict__ tg) {
+    ISTI_CONST_TABLE(t, tg);
+    const XcdRange r = xcd_range(t->clear_blocks);
+    for (uint32_t k = r.first; k < r.end; k += r.stride) {
+        const uint32_t j = find_owner(t->clear_begin, t->n_jobs, k);
+        clear_pairs(t->jobs[j].splat, t->jobs[j].n_px, (k - t->clear_begin[j]) * 256u + threadIdx.x);
+    }
+}
+
+__global__ __launch_bounds__(256) void splat_batch_kernel(const BatchTable *__restrict__ tg) {
+    ISTI_CONST_TABLE(t, tg);
+    const XcdRange r = xcd_range(t->splat_blocks);
+    for (uint32_t k = r.first; k < r.end; k += r.stride) {
+        const uint32_t j = find_owner(t->splat_begin, t->n_jobs, k);
+#if defined(__HIP_DEVICE_COMPILE__)
+        const KernArg<SplatJob> &J = *as_const(&t->jobs[j]);
+        const KernArg<BatchRequest> &R = *as_const(&t->req[J.req]);
+#else
+        const SplatJob &J = t->jobs[j];
+        const BatchRequest &R = t->req[J.req];
+#endif
+        splat_pixel(J, R.vp_cam, R.vp_main[J.main_slot], R.init_linear != 0,
+                    (k - t->splat_begin[j]) * SPLAT_BLOCK + threadIdx.x);
+    }
+}
+
+template <int W, bool MAV, int NAUX>
+__global__ __launch_bounds__(256) void resolve_batch_kernel(const BatchTable *__restrict__ tg, int variant) {
+    __shared__ ResolveLds<W, NAUX> lds;
+    ISTI_CONST_TABLE(t, tg);
+    const uint32_t r0 = t->var_req_begin[variant], r1 = t->var_req_begin[variant + 1];
+    const uint32_t base = t->tile_begin[r0];
+    const XcdRange r = xcd_range(t->tile_begin[r1] - base);
+    for (uint32_t k = r.first; k < r.end; k += r.stride) {
+        const uint32_t q = r0 + find_owner(t->tile_begin + r0, r1 - r0, base + k);
+#if defined(__HIP_DEVICE_COMPILE__)
+        const KernArg<ResolveArgs> &a = *as_const(&t->req[q].args);
+#else
+        const ResolveArgs &a = t->req[q].args;
+#endif
+        resolve_tile<W, MAV, NAUX>(a, base + k - t->tile_begin[q], lds);
+        __syncthreads();  // the next tile restages LDS
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Deterministic mean z. Row sums: one wave per row, lane l accumulates
 // columns l, l+64, ... in double, lanes folded by halving. Mean: one block,
@@ -627,6 +734,45 @@ hipError_t launch_resolve(const ResolveArgs &a, int window, bool always_valid, h
     if (window == 1 && !always_valid) return launch_resolve_n<1, false>(a, s);
     if (window == 3 && always_valid) return launch_resolve_n<3, true>(a, s);
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_clear_batch(const BatchTable *t, uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(clear_batch_kernel, dim3(grid), dim3(256), 0, s, t);
+    return hipGetLastError();
+}
+
+hipError_t launch_splat_batch(const BatchTable *t, uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(splat_batch_kernel, dim3(grid), dim3(SPLAT_BLOCK), 0, s, t);
+    return hipGetLastError();
+}
+
+// Kernel of each resolve variant: (window 1, plain) or (window 3, main always valid) x n_aux.
+template <int V>
+static void *resolve_batch_fn() {
+    constexpr bool mav = V > ISTI_MAX_AUX;
+    constexpr int n_aux = mav ? V - 1 - ISTI_MAX_AUX : V;
+    return reinterpret_cast<void *>(&resolve_batch_kernel<mav ? 3 : 1, mav, n_aux>);
+}
+static void *resolve_batch_ptr(int v) {
+    static void *const table[N_VARIANTS] = {resolve_batch_fn<0>(), resolve_batch_fn<1>(), resolve_batch_fn<2>(),
+                                            resolve_batch_fn<3>(), resolve_batch_fn<4>(), resolve_batch_fn<5>(),
+                                            resolve_batch_fn<6>(), resolve_batch_fn<7>(), resolve_batch_fn<8>(),
+                                            resolve_batch_fn<9>()};
+    return (v >= 0 && v < N_VARIANTS) ? table[v] : nullptr;
+}
+
+hipError_t launch_resolve_batch(const BatchTable *t, int variant, uint32_t grid, hipStream_t s) {
+    void *fn = resolve_batch_ptr(variant);
+    if (!fn) return hipErrorInvalidValue;
+    void *args[] = {&t, &variant};
+    return hipLaunchKernel(fn, dim3(grid), dim3(TILE_W * TILE_H), args, 0, s);
+}
+
+int resolve_batch_blocks_per_cu(int variant) {
+    void *fn = resolve_batch_ptr(variant);
+    int n = 0;
+    if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, TILE_W * TILE_H, 0) != hipSuccess) return 1;
+    return n > 0 ? n : 1;
 }
 
 hipError_t launch_mean_z(const float *z, uint32_t w, uint32_t h, double *row_sums, float *mean, hipStream_t s) {

@@ -28,6 +28,8 @@ struct SplatJob {
     uint32_t w, h, n_px, block_begin;
     float weight;
     int32_t main_slot;  // which main view's VP decides visible_in_main
+    int32_t req;        // owning request (batched launches)
+    int32_t pad;
     double inv_w, inv_h;  // RN64(1/w), RN64(1/h) for pixel2CS
     Mat4 inv_vp_from;
     Mat4 vp_to;
@@ -94,6 +96,40 @@ struct TimewarpArgs {
     int32_t aux_set;
     uint32_t tiles_x, n_tiles;
 };
+
+// ---------------------------------------------------------------------------
+// Batched renders: a device-resident work table read by persistent kernels.
+// ---------------------------------------------------------------------------
+constexpr int N_VARIANTS = 2 * (1 + ISTI_MAX_AUX);  // (window 1 | window 3 + always-valid) x n_aux
+__host__ __device__ constexpr int resolve_variant(bool mav, int n_aux) { return (mav ? 1 + ISTI_MAX_AUX : 0) + n_aux; }
+
+struct BatchRequest {
+    ResolveArgs args;
+    Mat4 vp_cam;
+    Mat4 vp_main[2];
+    int32_t init_linear;
+    int32_t pad[3];
+};
+
+// Header of one batch; the arrays it points to live in the same device
+// allocation, right behind it.
+struct BatchTable {
+    uint32_t n_req, n_jobs;
+    uint32_t clear_blocks, splat_blocks;
+    uint32_t var_req_begin[N_VARIANTS + 1];  // requests are ordered by resolve variant
+    uint32_t pad;
+    const BatchRequest *req;       // [n_req]
+    const SplatJob *jobs;          // [n_jobs]
+    const uint32_t *clear_begin;   // [n_jobs + 1] block prefix of the clear stage
+    const uint32_t *splat_begin;   // [n_jobs + 1] block prefix of the splat stage
+    const uint32_t *tile_begin;    // [n_req + 1] tile prefix of the resolve stage
+};
+
+hipError_t launch_clear_batch(const BatchTable *t, uint32_t grid, hipStream_t s);
+hipError_t launch_splat_batch(const BatchTable *t, uint32_t grid, hipStream_t s);
+hipError_t launch_resolve_batch(const BatchTable *t, int variant, uint32_t grid, hipStream_t s);
+// blocks of one resolve variant that fit on one CU at once
+int resolve_batch_blocks_per_cu(int variant);
 
 hipError_t launch_clear(const ClearArgs &a, hipStream_t s);
 hipError_t launch_splat(const SplatArgs &a, hipStream_t s);

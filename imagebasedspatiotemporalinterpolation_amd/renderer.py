@@ -224,3 +224,58 @@ class Renderer:
             "cam_ff": N.device_to_host(sv.cam_ff, n * 16, np.float32, (sv.height, sv.width, 4)),
             "full_ff": N.device_to_host(sv.full_ff, n * 16, np.float32, (sv.height, sv.width, 4)),
         }
+
+
+class Batch:
+    """Many renders in one persistent launch per stage (isti_render_batch).
+
+    Each request is (renderer, RenderInfo, render_buffer); a renderer may
+    appear once per call and must have its keyframes set. Results equal
+    separate Renderer.render calls. graph=True replays the stage sequence
+    from captured hipGraphs.
+    """
+
+    def __init__(self, max_requests: int = 64, device: int = 0, graph: bool = False):
+        h = ctypes.c_void_p()
+        st = N.lib().isti_batch_create(int(device), int(max_requests), ctypes.byref(h))
+        if st != N.ISTI_OK:
+            raise N.IstiError(st, N.lib().isti_last_error(None).decode())
+        self._h = h
+        self.max_requests = max_requests
+        self.set_graph(graph)
+
+    def set_graph(self, enable: bool) -> None:
+        self._check(N.lib().isti_batch_set_graph(self._h, 1 if enable else 0))
+
+    def _check(self, st):
+        if st != N.ISTI_OK:
+            raise N.IstiError(st, N.lib().isti_batch_last_error(self._h).decode())
+
+    def render(self, requests, stream=None) -> None:
+        """requests: iterable of (Renderer, RenderInfo, render_buffer tensor)."""
+        import torch
+
+        reqs = list(requests)
+        n = len(reqs)
+        rs = (ctypes.c_void_p * max(n, 1))()
+        infos = (N.RenderInfoC * max(n, 1))()
+        outs = (ctypes.c_void_p * max(n, 1))()
+        pitches = (ctypes.c_size_t * max(n, 1))()
+        for i, (r, info, buf) in enumerate(reqs):
+            rs[i] = r._h.value
+            infos[i] = info.to_c()
+            outs[i] = buf.data_ptr()
+            pitches[i] = int(buf.shape[1]) * 4
+        s = stream if stream is not None else torch.cuda.current_stream()
+        self._check(N.lib().isti_render_batch(self._h, rs, infos, outs, pitches, n, ctypes.c_void_p(s.cuda_stream)))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            N.lib().isti_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -180,6 +180,36 @@ isti_status isti_stage_times(isti_renderer *r, float ms[3]);
  * since timing was enabled, and their count. Waits for pending renders. */
 isti_status isti_stage_totals(isti_renderer *r, double total_ms[3], int64_t *count);
 
+/* ---- Batched rendering (no counterpart in the reference, which renders one
+ * view per call with a host sync inside, src/renderer.cu:650). Serves many
+ * renders at once -- client sessions, stereo eyes, views of uneven size --
+ * with ONE persistent launch per stage: splat clear, reproject + splat, and
+ * one resolve launch per distinct (mode group, n_aux) present. Work
+ * descriptors go to a device table (double-buffered, one async copy per
+ * batch), so the kernel grids and arguments are the same for every batch and
+ * the stage sequence can be replayed from a captured hipGraph. */
+typedef struct isti_batch isti_batch;
+
+/* A batch context for up to max_requests renders per call on `device`. */
+isti_status isti_batch_create(int32_t device, int32_t max_requests, isti_batch **out);
+isti_status isti_batch_destroy(isti_batch *b);
+
+/* enable != 0: launch the stages through captured hipGraphs (one per set of
+ * resolve variants in a batch, captured on first use). */
+isti_status isti_batch_set_graph(isti_batch *b, int32_t enable);
+
+/* Renders request i = (renderers[i], infos[i]) into outs[i] (row pitch
+ * out_pitches[i] bytes) for i < n, stream-ordered on `stream`, with the same
+ * results as n separate isti_render calls. Modes 0-3 (timewarp is not
+ * batched). A renderer may appear at most once per call; its keyframes must
+ * be set. Waits on the host only when both table buffers are still in use by
+ * earlier batches. */
+isti_status isti_render_batch(isti_batch *b, isti_renderer *const *renderers, const isti_render_info *infos,
+                              uint8_t *const *outs, const size_t *out_pitches, int32_t n, void *stream);
+
+/* Message of the last failure of a batch call. Never NULL. */
+const char *isti_batch_last_error(const isti_batch *b);
+
 /* Message of the last failure of `r`, or of the calling thread's last
  * isti_create failure when r is NULL. Never NULL. */
 const char *isti_last_error(const isti_renderer *r);
