@@ -45,6 +45,9 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-baseline sample budget")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--traffic-file", default=str(ROOT / "profiles" / "traffic_1080p.json"))
+    p.add_argument("--serving-seconds", type=float, default=2.0,
+                   help="duration of the 48-session serving mix (0 disables it)")
+    p.add_argument("--stereo-frames", type=int, default=60, help="stereo 2x2064x2208 frames timed (0 disables)")
     p.add_argument("--copy-probe", action="store_true",
                    help="only run a plain 256 MiB device copy 3 times (byte-counter calibration)")
     return p.parse_args()
@@ -100,6 +103,60 @@ def cpu_baseline(fwd, bwd, infos, budget_s: float):
         "kind": "port",
         "sample": f"{done} of the 7 intermediate 1920x1080 frames of one keyframe pair (same inputs), "
                   f"oracle with {threads} OpenMP threads, {dt:.1f} s",
+    }
+
+
+def stereo_bench(n_frames: int, seed: int) -> dict:
+    """Stereo HMD workload: both 2064x2208 eyes of a frame in one batched launch per stage."""
+    import torch
+
+    from imagebasedspatiotemporalinterpolation_amd import renderer as R
+    from imagebasedspatiotemporalinterpolation_amd import scene
+
+    cfg = scene.defaults(3, seed)
+    fpa = cfg.keyframe_spacing
+    eyes = []
+    for e in range(2):
+        fwd, bwd = scene.keyframe(cfg, 0, e, +1), scene.keyframe(cfg, 1, e, -1)
+        r = R.Renderer(n_aux=0)
+        fd = R.FrameData(R.KeyframeView.from_numpy(fwd), R.KeyframeView.from_numpy(bwd))
+        r.set_frame(fd)
+        out = torch.zeros((cfg.height, cfg.width, 4), dtype=torch.uint8, device="cuda")
+        eyes.append((r, fd, out))
+    infos = []
+    for b in range(1, fpa):
+        per_eye = []
+        for e in range(2):
+            vp, ivp, _, _ = scene.camera(cfg, b, e)
+            per_eye.append(R.RenderInfo((cfg.width, cfg.height), b / fpa, vp, ivp,
+                                        R.AdditionalRenderInfo(fpa, 0, False, R.RenderMode.OURS)))
+        infos.append(per_eye)
+    batch = R.Batch(max_requests=2, graph=True)
+    stream = torch.cuda.current_stream()
+
+    def frame(i):
+        batch.render([(eyes[e][0], infos[i % len(infos)][e], eyes[e][2]) for e in range(2)], stream=stream)
+
+    for i in range(10):
+        frame(i)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n_frames)]
+    t0 = time.perf_counter()
+    for i in range(n_frames):
+        ev[i][0].record(stream)
+        frame(i)
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = np.array([a.elapsed_time(b) for a, b in ev])
+    batch.close()
+    return {
+        "workload": "stereo 2x2064x2208 (IPD 63 mm), 64 spheres on sinusoidal paths, 0.15 m + 2 deg per frame, "
+                    "fpa 6, mode 0, both eyes per batched launch (hipGraph)",
+        "mpix_per_s": round(n_frames * 2 * cfg.width * cfg.height / wall / 1e6, 1),
+        "p50_frame_ms": round(float(np.percentile(ms, 50)), 4),
+        "p99_frame_ms": round(float(np.percentile(ms, 99)), 4),
+        "frames": n_frames,
     }
 
 
@@ -224,6 +281,17 @@ def main():
             "traffic": traffic,
         },
     }
+    if args.stereo_frames > 0:
+        print("[bench] stereo workload", file=sys.stderr, flush=True)
+        res["stereo"] = stereo_bench(args.stereo_frames, args.seed)
+    if args.serving_seconds > 0:
+        from imagebasedspatiotemporalinterpolation_amd import serving
+
+        print("[bench] serving mix", file=sys.stderr, flush=True)
+        res["serving"] = serving.run_mix(duration_s=args.serving_seconds, seed=args.seed, n_streams=3,
+                                         max_batch=16, graph=True)
+        res["serving"]["workload"] = ("48 sessions from {640x360, 1280x720, 1920x1080, stereo 2x2064x2208} "
+                                      "(weights .3/.3/.25/.15), Poisson arrivals 90 Hz per session, mode 0")
     if not args.no_cpu_baseline:
         print("[bench] timing the CPU oracle sample", file=sys.stderr, flush=True)
         res["cpu_baseline"] = cpu_baseline(fwd, bwd, infos, args.cpu_seconds)
