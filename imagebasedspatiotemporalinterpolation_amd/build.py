@@ -61,7 +61,7 @@ def build_native(verbose: bool = False) -> None:
     obj_dir = PKG / "_build"
     obj_dir.mkdir(exist_ok=True)
     objs = []
-    for src in ("isti_kernels.hip", "isti_api.cpp"):
+    for src in ("isti_kernels.hip", "isti_api.cpp", "unet.hip"):
         obj = obj_dir / (src.rsplit(".", 1)[0] + ".o")
         extra = ["-x", "hip"] if src.endswith(".hip") else []
         _run([HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", *extra, "-c", CSRC / src, "-o", obj], verbose)

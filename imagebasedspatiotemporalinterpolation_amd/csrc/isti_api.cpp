@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -20,6 +21,7 @@
 #include <vector>
 
 #include "../../include/isti.h"
+#include "../../include/isti_learned.h"
 #include "isti_kernels.hpp"
 
 using namespace isti;
@@ -61,6 +63,7 @@ struct isti_renderer {
     float *mean_z = nullptr;
     bool timing = false;
     TimingRing ring;
+    uint32_t *q_out = nullptr;  // optional search-result capture (isti_set_search_capture)
     std::string err;
 };
 
@@ -322,7 +325,20 @@ isti_status plan_frame(isti_renderer *r, const isti_render_info *info, uint8_t *
     ra.max_iter = info->max_search_iterations >= 0 ? info->max_search_iterations : iters;
     ra.tiles_x = (ra.w + TILE_W - 1) / TILE_W;
     ra.n_tiles = ra.tiles_x * ((ra.h + TILE_H - 1) / TILE_H);
+    ra.q_out = r->q_out;
     return ISTI_OK;
+}
+
+// Smallest power-of-two reduction of the render size that fits the network grid.
+bool net_shift_for(uint32_t w, uint32_t h, uint32_t net_w, uint32_t net_h, uint32_t &shift) {
+    for (uint32_t k = 0; k <= 4; k++) {
+        const uint32_t cw = (w + (1u << k) - 1) >> k, ch = (h + (1u << k) - 1) >> k;
+        if (cw <= net_w && ch <= net_h) {
+            shift = k;
+            return true;
+        }
+    }
+    return false;
 }
 
 constexpr size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
@@ -558,6 +574,99 @@ isti_status isti_render_batch(isti_batch *b, isti_renderer *const *rs, const ist
     b->in_use[sl] = true;
     b->slot = 1 - sl;
     return ISTI_OK;
+}
+
+isti_status isti_set_search_capture(isti_renderer *r, uint32_t *q_out) {
+    if (!r) return ISTI_ERR_INVALID_ARG;
+    r->q_out = q_out;
+    return ISTI_OK;
+}
+
+isti_status isti_learned_features(isti_renderer *r, const isti_render_info *info, uint16_t *x, uint32_t net_w,
+                                  uint32_t net_h, void *stream) {
+    if (!r) return ISTI_ERR_INVALID_ARG;
+    if (!info || !x || net_w == 0 || net_h == 0) return fail(r, ISTI_ERR_INVALID_ARG, "bad features arguments");
+    FeatureArgs fa;
+    std::memset(&fa, 0, sizeof(fa));
+    if (!net_shift_for(info->width, info->height, net_w, net_h, fa.net_shift))
+        return fail(r, ISTI_ERR_UNSUPPORTED_SHAPE, "network grid too small for the render size");
+    for (int sl = 0; sl < 2; sl++) {
+        const Scratch &sc = r->scratch[sl][0];
+        if (!sc.cam_ff || sc.w != info->width || sc.h != info->height || !r->view_set[sl][0])
+            return fail(r, ISTI_ERR_INVALID_ARG, "render this frame before building its features");
+        fa.v[sl] = resolve_view(r->views[sl][0], sc, sl == 0, 0);
+    }
+    fa.x = x;
+    fa.w = info->width;
+    fa.h = info->height;
+    fa.net_w = net_w;
+    fa.net_h = net_h;
+    fa.alpha = info->alpha;
+    const hipError_t e = launch_features(fa, reinterpret_cast<hipStream_t>(stream));
+    return e == hipSuccess ? ISTI_OK : dev_fail(r, e, "features launch");
+}
+
+isti_status isti_learned_loss(const float *out, const uint32_t *q_target, uint32_t width, uint32_t height,
+                              uint32_t net_w, uint32_t net_h, float conf_weight, float *dout, float *loss_acc,
+                              void *stream) {
+    uint32_t shift = 0;
+    if (!out || !q_target || !dout || !loss_acc || !net_shift_for(width, height, net_w, net_h, shift))
+        return ISTI_ERR_INVALID_ARG;
+    const hipError_t e = launch_learned_loss(out, q_target, width, height, net_w, net_h, shift, conf_weight, dout,
+                                             loss_acc, reinterpret_cast<hipStream_t>(stream));
+    return e == hipSuccess ? ISTI_OK : ISTI_ERR_DEVICE;
+}
+
+isti_status isti_render_learned(isti_renderer *r, const isti_render_info *info, const float *net_out,
+                                uint32_t net_w, uint32_t net_h, float conf_threshold, uint32_t *stats,
+                                uint8_t *out, size_t out_pitch, void *stream) {
+    if (!r) return ISTI_ERR_INVALID_ARG;
+    if (!info || !out || !net_out) return fail(r, ISTI_ERR_INVALID_ARG, "info, out and net_out must be non-null");
+    if (!(conf_threshold > 0.0f && conf_threshold < 1.0f))
+        return fail(r, ISTI_ERR_INVALID_ARG, "conf_threshold must be in (0, 1)");
+    isti_status st = validate(r, info, out_pitch);
+    if (st != ISTI_OK) return st;
+    if (info->mode == ISTI_MODE_TIMEWARP || info->mode == ISTI_MODE_BACKWARD_BIDIRECTIONAL)
+        return fail(r, ISTI_ERR_INVALID_ARG, "learned starts apply to modes 0, 2 and 3");
+    hipError_t e = hipSetDevice(r->device);
+    if (e != hipSuccess) return dev_fail(r, e, "hipSetDevice");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    FramePlan P;
+    st = plan_frame(r, info, out, out_pitch, P);
+    if (st != ISTI_OK) return st;
+    if (!net_shift_for(info->width, info->height, net_w, net_h, P.ra.net_shift))
+        return fail(r, ISTI_ERR_UNSUPPORTED_SHAPE, "network grid too small for the render size");
+    P.ra.net_out = net_out;
+    P.ra.net_w = net_w;
+    P.ra.net_h = net_h;
+    P.ra.conf_logit = std::log(conf_threshold / (1.0f - conf_threshold));
+    P.ra.learned_stats = stats;
+    ClearArgs ca;
+    std::memset(&ca, 0, sizeof(ca));
+    SplatArgs sa;
+    std::memset(&sa, 0, sizeof(sa));
+    sa.vp_cam = P.vp_cam;
+    sa.vp_main[0] = P.vp_main[0];
+    sa.vp_main[1] = P.vp_main[1];
+    sa.init_linear = P.init_linear;
+    uint32_t cb = 0, sb = 0;
+    for (int j = 0; j < P.nj; j++) {
+        sa.job[j] = P.jobs[j];
+        ca.buf[j] = P.jobs[j].splat;
+        ca.n[j] = P.jobs[j].n_px;
+        ca.block_begin[j] = cb;
+        cb += (P.jobs[j].n_px + 511u) / 512u;
+        sa.job[j].block_begin = sb;
+        sb += (P.jobs[j].n_px + SPLAT_BLOCK - 1) / SPLAT_BLOCK;
+    }
+    ca.n_bufs = sa.n_jobs = P.nj;
+    ca.n_blocks = cb;
+    sa.n_blocks = sb;
+    if ((e = mark(r, 0, s)) == hipSuccess) e = launch_clear(ca, s);
+    if (e == hipSuccess && (e = mark(r, 1, s)) == hipSuccess) e = launch_splat(sa, s);
+    if (e == hipSuccess && (e = mark(r, 2, s)) == hipSuccess) e = launch_resolve_learned(P.ra, s);
+    if (e == hipSuccess) e = mark(r, 3, s);
+    return e == hipSuccess ? ISTI_OK : dev_fail(r, e, "learned render launch");
 }
 
 const char *isti_batch_last_error(const isti_batch *b) { return b ? b->err.c_str() : g_create_error.c_str(); }

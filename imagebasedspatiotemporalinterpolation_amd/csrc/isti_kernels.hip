@@ -369,7 +369,7 @@ struct ResolveLds {
 
 // One 16x16 tile of one render. Every thread of the block must call it (the
 // LDS staging ends in a barrier); pixels outside the image return early.
-template <int W, bool MAV, int NAUX, typename A>
+template <int W, bool MAV, int NAUX, bool LEARNED = false, typename A>
 __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveLds<W, NAUX> &lds) {
     constexpr int NC = 1 + NAUX;
     constexpr int SPAN = TILE_W + 2 * W;
@@ -431,10 +431,30 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
     // 2. searches in lock-step; the fwd search is skipped for alpha >= 1 (:335)
     Chain ch[NC][2];
     bool ran[NC][2];
+    const SplatInfo window0[2] = {si[0][0], si[0][1]};  // main camera's window starts
+    bool learned[2] = {false, false};
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         ran[c][0] = si[c][0].depth > 0.0f && fc.alpha < 1.0f;
         ran[c][1] = si[c][1].depth > 0.0f;
+        if (LEARNED && c == 0) {
+            // learned start: the network's offset from this pixel, where it is confident
+#pragma unroll
+            for (int d = 0; d < 2; d++) {
+                if (d == 0 && !(fc.alpha < 1.0f)) continue;
+                const ResolveView &v = d ? a.bwd[0] : a.fwd[0];
+                const float *o = a.net_out + (((size_t)d * a.net_h + (y >> a.net_shift)) * a.net_w + (x >> a.net_shift)) * 16;
+                if (o[2] >= a.conf_logit) {
+                    float fx = (float)x + o[0] + 0.5f, fy = (float)y + o[1] + 0.5f;
+                    fx = fx >= 0.0f ? (fx <= (float)(v.w - 1) ? fx : (float)(v.w - 1)) : 0.0f;  // NaN -> 0
+                    fy = fy >= 0.0f ? (fy <= (float)(v.h - 1) ? fy : (float)(v.h - 1)) : 0.0f;
+                    si[0][d].x = (uint32_t)fx;
+                    si[0][d].y = (uint32_t)fy;
+                    ran[0][d] = true;
+                    learned[d] = true;
+                }
+            }
+        }
         chain_start(ch[c][0], ran[c][0], si[c][0].x, si[c][0].y, a.fwd[c]);
         chain_start(ch[c][1], ran[c][1], si[c][1].x, si[c][1].y, a.bwd[c]);
     }
@@ -469,6 +489,61 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
                 vis[c][d] = m.w > 0.0f;
             }
         }
+    }
+    if (LEARNED) {
+        // the plain search judges the network: a learned start whose search
+        // does not end below EPS_1 is redone from the window start
+        bool redo[2];
+#pragma unroll
+        for (int d = 0; d < 2; d++) {
+            redo[d] = learned[d] && !(err[0][d] < EPS_1_FULL_RES);
+            if (redo[d]) {
+                si[0][d] = window0[d];
+                ran[0][d] = window0[d].depth > 0.0f && (d == 1 || fc.alpha < 1.0f);
+            }
+            chain_start(ch[0][d], redo[d] && ran[0][d], si[0][d].x, si[0][d].y, d ? a.bwd[0] : a.fwd[0]);
+        }
+        for (int i = 0; i < a.max_iter; i++) {
+            if (!(ch[0][0].active || ch[0][1].active)) break;
+            chain_step(ch[0][0], i, a.max_iter, pcs, a.fwd[0]);
+            chain_step(ch[0][1], i, a.max_iter, pcs, a.bwd[0]);
+        }
+#pragma unroll
+        for (int d = 0; d < 2; d++) {
+            if (!redo[d]) continue;
+            const ResolveView &v = d ? a.bwd[0] : a.fwd[0];
+            err[0][d] = FLT_MAX;
+            vis[0][d] = false;
+            if (ran[0][d]) {
+                const uint32_t qx = ch[0][d].h0 & 0xFFFFu, qy = ch[0][d].h0 >> 16;
+                const float4 m = ch[0][d].m;
+                si[0][d].x = qx;
+                si[0][d].y = qy;
+                si[0][d].depth = z_to_depth(at(v.z, v.w, qx, qy) + m.z);
+                err[0][d] = reproj_error(make_uint2(qx, qy), v, cpcs[0], m);
+                vis[0][d] = m.w > 0.0f;
+            }
+        }
+        if (a.learned_stats) {  // one atomic per wave and counter (ballots see active lanes only)
+            const bool fwd_ok = fc.alpha < 1.0f;
+            const uint32_t kept = (uint32_t)(__popcll(__ballot(learned[0] && !redo[0])) +
+                                             __popcll(__ballot(learned[1] && !redo[1])));
+            const uint32_t rej = (uint32_t)(__popcll(__ballot(redo[0])) + __popcll(__ballot(redo[1])));
+            const uint32_t low = (uint32_t)(__popcll(__ballot(fwd_ok && !learned[0])) + __popcll(__ballot(!learned[1])));
+            const uint32_t tot = (uint32_t)(__popcll(__ballot(fwd_ok)) + __popcll(__ballot(1)));
+            const unsigned long long act = __ballot(1);
+            if ((int)(threadIdx.x & 63) == __ffsll((long long)act) - 1) {
+                atomicAdd(&a.learned_stats[0], kept);
+                atomicAdd(&a.learned_stats[1], rej);
+                atomicAdd(&a.learned_stats[2], low);
+                atomicAdd(&a.learned_stats[3], tot);
+            }
+        }
+    }
+    if (a.q_out) {  // capture of the main camera's search results
+#pragma unroll
+        for (int d = 0; d < 2; d++)
+            a.q_out[2 * (y * a.w + x) + d] = err[0][d] < EPS_1_FULL_RES ? pack_px(si[0][d].x, si[0][d].y) : 0xffffffffu;
     }
     // 3.-4. per-camera decision (:366-396), folded at once into the running
     // aux selection (:429-451) so no per-camera array outlives the loop
@@ -529,6 +604,102 @@ __global__ __launch_bounds__(256) void resolve_kernel(const ResolveArgs args) {
 #endif
     __shared__ ResolveLds<W, NAUX> lds;
     resolve_tile<W, MAV, NAUX>(a, xcd_remap(blockIdx.x, a.n_tiles), lds);
+}
+
+// Learned search starts: the same resolve, with the main camera's searches
+// starting where the network says (see resolve_tile, LEARNED).
+template <int NAUX>
+__global__ __launch_bounds__(256) void resolve_learned_kernel(const ResolveArgs args) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const KernArg<ResolveArgs> &a = *kernarg_ptr<ResolveArgs>();
+#else
+    const ResolveArgs &a = args;
+#endif
+    __shared__ ResolveLds<1, NAUX> lds;
+    resolve_tile<1, false, NAUX, true>(a, xcd_remap(blockIdx.x, a.n_tiles), lds);
+}
+
+// ---------------------------------------------------------------------------
+// Network input: 15 features per cell and direction (see isti_learned.h),
+// sampled at the cell's top-left full-resolution pixel; channel 15 and cells
+// outside the image are zero.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void features_kernel(const FeatureArgs a) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t cells = a.net_w * a.net_h;
+    if (i >= 2u * cells) return;
+    const uint32_t d = i / cells, c = i - d * cells;
+    const uint32_t cy = c / a.net_w, cx = c - cy * a.net_w;
+    const uint32_t x = cx << a.net_shift, y = cy << a.net_shift;
+    float f[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) f[k] = 0.0f;
+    if (x < a.w && y < a.h) {
+        const ResolveView &own = a.v[d], &oth = a.v[1 - d];
+        const uint32_t p = y * a.w + x;
+        const uchar4 co = own.img[p], cq = oth.img[p];
+        const float4 mo = own.cam_ff[p], mq = oth.cam_ff[p];
+        const float hw = 0.5f * (float)a.w, hh = 0.5f * (float)a.h;
+        f[0] = co.x / 255.0f; f[1] = co.y / 255.0f; f[2] = co.z / 255.0f;
+        f[3] = cq.x / 255.0f; f[4] = cq.y / 255.0f; f[5] = cq.z / 255.0f;
+        f[6] = own.z[p]; f[7] = oth.z[p];
+        f[8] = mo.x * hw; f[9] = mo.y * hh; f[10] = mo.z;
+        f[11] = mq.x * hw; f[12] = mq.y * hh; f[13] = mq.z;
+        f[14] = d == 0 ? a.alpha : 1.0f - a.alpha;
+    }
+    uint16_t *o = a.x + (size_t)i * 16;
+#pragma unroll
+    for (int k = 0; k < 16; k += 2) {
+        uint32_t u0 = __float_as_uint(f[k]), u1 = __float_as_uint(f[k + 1]);
+        u0 = (u0 + 0x7fffu + ((u0 >> 16) & 1u)) >> 16;  // round to nearest even (finite inputs)
+        u1 = (u1 + 0x7fffu + ((u1 >> 16) & 1u)) >> 16;
+        reinterpret_cast<uint32_t *>(o)[k / 2] = u0 | (u1 << 16);
+    }
+}
+
+// Training loss against captured plain-search results (see isti_learned.h).
+__global__ __launch_bounds__(256) void learned_loss_kernel(const float *__restrict__ out, const uint32_t *__restrict__ q,
+                                                           uint32_t w, uint32_t h, uint32_t net_w, uint32_t net_h,
+                                                           uint32_t shift, float conf_weight, float *dout,
+                                                           float *loss_acc) {
+    __shared__ float red[256];
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t cells = net_w * net_h;
+    float loss = 0.0f;
+    if (i < 2u * cells) {
+        const uint32_t d = i / cells, c = i - d * cells;
+        const uint32_t cy = c / net_w, cx = c - cy * net_w;
+        const uint32_t x = cx << shift, y = cy << shift;
+        const float *o = out + (size_t)i * 16;
+        float *g = dout + (size_t)i * 16;
+#pragma unroll
+        for (int k = 0; k < 16; k++) g[k] = 0.0f;
+        if (x < w && y < h) {
+            const float norm = 1.0f / (2.0f * (float)cells);
+            const uint32_t t = q[2 * (y * w + x) + d];
+            const bool valid = t != 0xffffffffu;
+            if (valid) {
+                const float tx = (float)(t & 0xFFFFu) - (float)x, ty = (float)(t >> 16) - (float)y;
+                const float ex = o[0] - tx, ey = o[1] - ty;
+                const float ax = fabsf(ex), ay = fabsf(ey);
+                loss += (ax < 1.0f ? 0.5f * ex * ex : ax - 0.5f) + (ay < 1.0f ? 0.5f * ey * ey : ay - 0.5f);
+                g[0] = norm * fmaxf(-1.0f, fminf(1.0f, ex));
+                g[1] = norm * fmaxf(-1.0f, fminf(1.0f, ey));
+            }
+            const float z = o[2], c01 = valid ? 1.0f : 0.0f;
+            const float sp = z > 0.0f ? z + log1pf(expf(-z)) : log1pf(expf(z));  // softplus
+            loss += conf_weight * (sp - c01 * z);
+            g[2] = norm * conf_weight * (1.0f / (1.0f + expf(-z)) - c01);
+            loss *= norm;
+        }
+    }
+    red[threadIdx.x] = loss;
+    __syncthreads();
+    for (int off = 128; off >= 1; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss_acc, red[0]);
 }
 
 // ---------------------------------------------------------------------------
@@ -734,6 +905,34 @@ hipError_t launch_resolve(const ResolveArgs &a, int window, bool always_valid, h
     if (window == 1 && !always_valid) return launch_resolve_n<1, false>(a, s);
     if (window == 3 && always_valid) return launch_resolve_n<3, true>(a, s);
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_features(const FeatureArgs &a, hipStream_t s) {
+    const uint32_t n = 2u * a.net_w * a.net_h;
+    hipLaunchKernelGGL(features_kernel, dim3((n + 255u) / 256u), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_learned_loss(const float *out, const uint32_t *q, uint32_t w, uint32_t h, uint32_t net_w,
+                               uint32_t net_h, uint32_t shift, float conf_weight, float *dout, float *loss_acc,
+                               hipStream_t s) {
+    const uint32_t n = 2u * net_w * net_h;
+    hipLaunchKernelGGL(learned_loss_kernel, dim3((n + 255u) / 256u), dim3(256), 0, s, out, q, w, h, net_w, net_h, shift,
+                       conf_weight, dout, loss_acc);
+    return hipGetLastError();
+}
+
+hipError_t launch_resolve_learned(const ResolveArgs &a, hipStream_t s) {
+    if (a.n_tiles == 0) return hipSuccess;
+    const dim3 grid(a.n_tiles), block(TILE_W * TILE_H);
+    switch (a.n_aux) {
+        case 0: hipLaunchKernelGGL(resolve_learned_kernel<0>, grid, block, 0, s, a); break;
+        case 1: hipLaunchKernelGGL(resolve_learned_kernel<1>, grid, block, 0, s, a); break;
+        case 2: hipLaunchKernelGGL(resolve_learned_kernel<2>, grid, block, 0, s, a); break;
+        case 3: hipLaunchKernelGGL(resolve_learned_kernel<3>, grid, block, 0, s, a); break;
+        default: hipLaunchKernelGGL(resolve_learned_kernel<4>, grid, block, 0, s, a); break;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_clear_batch(const BatchTable *t, uint32_t grid, hipStream_t s) {

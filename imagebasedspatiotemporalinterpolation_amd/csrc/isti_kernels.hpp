@@ -72,6 +72,12 @@ struct ResolveArgs {
     float alpha;
     int32_t fpa, sfpa, no_shadows, n_aux, max_iter;
     uint32_t tiles_x, n_tiles;
+    uint32_t *q_out;          // optional capture of the main camera's search results
+    // learned search starts (main camera): null when unused
+    const float *net_out;     // [2][net_h][net_w][16]
+    uint32_t net_w, net_h, net_shift;
+    float conf_logit;         // starts are used where the confidence logit >= this
+    uint32_t *learned_stats;  // kept, rejected, low confidence, total
 };
 
 struct ClearArgs {
@@ -130,6 +136,18 @@ hipError_t launch_splat_batch(const BatchTable *t, uint32_t grid, hipStream_t s)
 hipError_t launch_resolve_batch(const BatchTable *t, int variant, uint32_t grid, hipStream_t s);
 // blocks of one resolve variant that fit on one CU at once
 int resolve_batch_blocks_per_cu(int variant);
+
+struct FeatureArgs {
+    ResolveView v[2];   // main fwd, main bwd (with this frame's cam_ff)
+    uint16_t *x;        // [2][net_h][net_w][16] bf16
+    uint32_t w, h, net_w, net_h, net_shift;
+    float alpha;
+};
+hipError_t launch_features(const FeatureArgs &a, hipStream_t s);
+hipError_t launch_learned_loss(const float *out, const uint32_t *q, uint32_t w, uint32_t h, uint32_t net_w,
+                               uint32_t net_h, uint32_t shift, float conf_weight, float *dout, float *loss_acc,
+                               hipStream_t s);
+hipError_t launch_resolve_learned(const ResolveArgs &a, hipStream_t s);
 
 hipError_t launch_clear(const ClearArgs &a, hipStream_t s);
 hipError_t launch_splat(const SplatArgs &a, hipStream_t s);
