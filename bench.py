@@ -48,6 +48,8 @@ def parse():
     p.add_argument("--serving-seconds", type=float, default=2.0,
                    help="duration of the 48-session serving mix (0 disables it)")
     p.add_argument("--stereo-frames", type=int, default=60, help="stereo 2x2064x2208 frames timed (0 disables)")
+    p.add_argument("--learned-frames", type=int, default=201,
+                   help="1080p frames timed for learned-init vs plain (0 disables)")
     p.add_argument("--copy-probe", action="store_true",
                    help="only run a plain 256 MiB device copy 3 times (byte-counter calibration)")
     return p.parse_args()
@@ -160,6 +162,83 @@ def stereo_bench(n_frames: int, seed: int) -> dict:
     }
 
 
+def learned_bench(n_frames: int, seed: int, iters: int) -> dict:
+    """Learned-init vs plain search on the C1 scene generator at 1920x1080
+    (fpa 4, mode 0, main camera only), the shipped network at 480x272."""
+    import torch
+
+    from imagebasedspatiotemporalinterpolation_amd import learned as Ln
+    from imagebasedspatiotemporalinterpolation_amd import renderer as R
+    from imagebasedspatiotemporalinterpolation_amd import scene
+
+    if not Ln.WEIGHTS.exists():
+        return {"error": f"no trained weights at {Ln.WEIGHTS.name}"}
+    params, meta = Ln.load_params(Ln.WEIGHTS)
+    W, H, shift = 1920, 1080, 2
+    cfg = scene.defaults(1, seed, width=W, height=H)
+    fpa = cfg.keyframe_spacing
+    fwd, bwd = scene.keyframe(cfg, 0, 0, +1), scene.keyframe(cfg, 1, 0, -1)
+    r = R.Renderer(0)
+    fd = R.FrameData(R.KeyframeView.from_numpy(fwd), R.KeyframeView.from_numpy(bwd))
+    r.set_frame(fd)
+    infos = []
+    for b in range(1, fpa):
+        vp, ivp, _, _ = scene.camera(cfg, b)
+        infos.append(R.RenderInfo((W, H), b / fpa, vp, ivp, R.AdditionalRenderInfo(fpa, 0, True, R.RenderMode.OURS),
+                                  iters))
+    li = Ln.LearnedInit(W, H, shift, params)
+    out = torch.zeros((H, W, 4), dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        for i in range(10):
+            fn(infos[i % len(infos)])
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n_frames)]
+        t0 = time.perf_counter()
+        for i in range(n_frames):
+            ev[i][0].record(stream)
+            fn(infos[i % len(infos)])
+            ev[i][1].record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms = np.array([a.elapsed_time(b) for a, b in ev])
+        return {"mpix_per_s": round(n_frames * W * H / wall / 1e6, 1),
+                "p50_frame_ms": round(float(np.percentile(ms, 50)), 4),
+                "p99_frame_ms": round(float(np.percentile(ms, 99)), 4)}
+
+    plain = timed(lambda ri: r.render(out, ri))
+    learned = timed(lambda ri: li.render(r, out, ri))
+    # the network alone (features already built for the last frame)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(20):
+        li.net.forward(li.x, li.out)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    net_ms = e0.elapsed_time(e1) / 20
+    li.close()
+    r.close()
+    ev = Ln.evaluate(params, W, H, shift, seeds=(seed + 900000,), pairs_per_seed=1, max_iter=iters)
+    return {
+        "workload": "C1 scene generator at 1920x1080 (checker plane + 16 spheres), fpa 4, mode 0, main camera; "
+                    "network trained on 4000 seeded 960x540 pairs, run at 480x272",
+        "plain": plain,
+        "learned": learned,
+        "speedup": round(learned["mpix_per_s"] / plain["mpix_per_s"], 3),
+        "network_ms": round(net_ms, 4),
+        "network_gflop": round(83.5e3 * 2 * li.net_w * li.net_h / 1e9, 1),
+        "iterations_saved_per_search": round(ev["iterations_saved_per_search"], 4),
+        "steps_per_search_plain": round(ev["steps_per_search_plain"], 4),
+        "steps_per_search_learned": round(ev["steps_per_search_learned"], 4),
+        "fallback_rate": round(ev["fallback_rate"], 4),
+        "learned_start_rate": round(ev["learned_start_rate"], 4),
+        "delta_psnr_db": round(ev["delta_psnr_db"], 4),
+        "psnr_learned_vs_plain_db": round(ev["psnr_learned_vs_plain"], 2),
+        "train_loss_last100": meta.get("loss_last100"),
+    }
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -269,7 +348,6 @@ def main():
         "p50_frame_ms": round(float(np.percentile(frame_ms, 50)), 4),
         "p99_frame_ms": round(float(np.percentile(frame_ms, 99)), 4),
         "stage_avg_ms": {k2: round(v2, 5) for k2, v2 in stage_avg_ms.items()},
-        "learned_init": None,
         "roofline": {
             "kernel": dom,
             "bound": "hbm",
@@ -281,6 +359,11 @@ def main():
             "traffic": traffic,
         },
     }
+    if args.learned_frames > 0:
+        print("[bench] learned-init vs plain search", file=sys.stderr, flush=True)
+        res["learned_init"] = learned_bench(args.learned_frames, args.seed, args.iters)
+    else:
+        res["learned_init"] = None
     if args.stereo_frames > 0:
         print("[bench] stereo workload", file=sys.stderr, flush=True)
         res["stereo"] = stereo_bench(args.stereo_frames, args.seed)

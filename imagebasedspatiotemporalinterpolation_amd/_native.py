@@ -116,6 +116,10 @@ def lib() -> ctypes.CDLL:
         raise RuntimeError(
             f"{LIB_PATH} is missing: build it with `python -m imagebasedspatiotemporalinterpolation_amd.build`"
         )
+    # torch's HIP runtime must be in the process first: libisti.so then binds
+    # to it (same soname) instead of loading a second runtime beside it.
+    import torch  # noqa: F401
+
     L = ctypes.CDLL(str(LIB_PATH))
     L.isti_create.argtypes = [POINTER(Config), POINTER(c_void_p)]
     L.isti_create.restype = c_int32

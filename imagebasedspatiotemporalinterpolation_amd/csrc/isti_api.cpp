@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <new>
 #include <string>
@@ -64,6 +65,9 @@ struct isti_renderer {
     bool timing = false;
     TimingRing ring;
     uint32_t *q_out = nullptr;  // optional search-result capture (isti_set_search_capture)
+    // frame whose splat stage isti_learned_features ran last (isti_render_learned resolves it)
+    isti_render_info prepared{};
+    bool prepared_valid = false;
     std::string err;
 };
 
@@ -137,6 +141,8 @@ isti_status validate(isti_renderer *r, const isti_render_info *info, size_t out_
         return fail(r, ISTI_ERR_INVALID_ARG, "mode must be 0..4");
     if (info->width == 0 || info->height == 0 || info->width > ISTI_MAX_DIM || info->height > ISTI_MAX_DIM)
         return fail(r, ISTI_ERR_UNSUPPORTED_SHAPE, "render resolution must be 1..65535 per axis");
+    if (info->max_search_iterations > ISTI_MAX_SEARCH_ITERATIONS)
+        return fail(r, ISTI_ERR_INVALID_ARG, "max_search_iterations must be <= 4096");
     if (out_pitch < (size_t)info->width * 4 || out_pitch % 4 != 0)
         return fail(r, ISTI_ERR_INVALID_ARG, "out_pitch_bytes must be a multiple of 4 and >= width*4");
     const int nv = info->mode == ISTI_MODE_TIMEWARP ? 1 : 1 + r->n_aux;
@@ -326,6 +332,41 @@ isti_status plan_frame(isti_renderer *r, const isti_render_info *info, uint8_t *
     ra.tiles_x = (ra.w + TILE_W - 1) / TILE_W;
     ra.n_tiles = ra.tiles_x * ((ra.h + TILE_H - 1) / TILE_H);
     ra.q_out = r->q_out;
+    return ISTI_OK;
+}
+
+// M1 clear and K1 reprojection + splat of every view of a planned frame
+// (src/renderer.cu:631-649), with the stage-timing marks 0, 1 and 2.
+isti_status splat_stage(isti_renderer *r, const FramePlan &P, hipStream_t s) {
+    ClearArgs ca;
+    std::memset(&ca, 0, sizeof(ca));
+    SplatArgs sa;
+    std::memset(&sa, 0, sizeof(sa));
+    sa.vp_cam = P.vp_cam;
+    sa.vp_main[0] = P.vp_main[0];
+    sa.vp_main[1] = P.vp_main[1];
+    sa.init_linear = P.init_linear;
+    uint32_t clear_blocks = 0, splat_blocks = 0;
+    for (int j = 0; j < P.nj; j++) {
+        SplatJob &J = sa.job[j];
+        J = P.jobs[j];
+        ca.buf[j] = J.splat;
+        ca.n[j] = J.n_px;
+        ca.block_begin[j] = clear_blocks;
+        clear_blocks += (J.n_px + 511u) / 512u;
+        J.block_begin = splat_blocks;
+        splat_blocks += (J.n_px + SPLAT_BLOCK - 1) / SPLAT_BLOCK;
+    }
+    ca.n_bufs = P.nj;
+    ca.n_blocks = clear_blocks;
+    sa.n_jobs = P.nj;
+    sa.n_blocks = splat_blocks;
+    hipError_t e;
+    if ((e = mark(r, 0, s)) != hipSuccess) return dev_fail(r, e, "event");
+    if ((e = launch_clear(ca, s)) != hipSuccess) return dev_fail(r, e, "clear launch");
+    if ((e = mark(r, 1, s)) != hipSuccess) return dev_fail(r, e, "event");
+    if ((e = launch_splat(sa, s)) != hipSuccess) return dev_fail(r, e, "splat launch");
+    if ((e = mark(r, 2, s)) != hipSuccess) return dev_fail(r, e, "event");
     return ISTI_OK;
 }
 
@@ -586,35 +627,50 @@ isti_status isti_learned_features(isti_renderer *r, const isti_render_info *info
                                   uint32_t net_h, void *stream) {
     if (!r) return ISTI_ERR_INVALID_ARG;
     if (!info || !x || net_w == 0 || net_h == 0) return fail(r, ISTI_ERR_INVALID_ARG, "bad features arguments");
+    isti_status st = validate(r, info, (size_t)info->width * 4);
+    if (st != ISTI_OK) return st;
+    if (info->mode == ISTI_MODE_TIMEWARP || info->mode == ISTI_MODE_BACKWARD_BIDIRECTIONAL)
+        return fail(r, ISTI_ERR_INVALID_ARG, "learned starts apply to modes 0, 2 and 3");
     FeatureArgs fa;
     std::memset(&fa, 0, sizeof(fa));
     if (!net_shift_for(info->width, info->height, net_w, net_h, fa.net_shift))
         return fail(r, ISTI_ERR_UNSUPPORTED_SHAPE, "network grid too small for the render size");
-    for (int sl = 0; sl < 2; sl++) {
-        const Scratch &sc = r->scratch[sl][0];
-        if (!sc.cam_ff || sc.w != info->width || sc.h != info->height || !r->view_set[sl][0])
-            return fail(r, ISTI_ERR_INVALID_ARG, "render this frame before building its features");
-        fa.v[sl] = resolve_view(r->views[sl][0], sc, sl == 0, 0);
-    }
+    hipError_t e = hipSetDevice(r->device);
+    if (e != hipSuccess) return dev_fail(r, e, "hipSetDevice");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    FramePlan P;
+    st = plan_frame(r, info, nullptr, (size_t)info->width * 4, P);
+    if (st != ISTI_OK) return st;
+    r->prepared_valid = false;
+    st = splat_stage(r, P, s);
+    if (st != ISTI_OK) return st;
+    for (int sl = 0; sl < 2; sl++) fa.v[sl] = sl == 0 ? P.ra.fwd[0] : P.ra.bwd[0];
     fa.x = x;
     fa.w = info->width;
     fa.h = info->height;
     fa.net_w = net_w;
     fa.net_h = net_h;
     fa.alpha = info->alpha;
-    const hipError_t e = launch_features(fa, reinterpret_cast<hipStream_t>(stream));
-    return e == hipSuccess ? ISTI_OK : dev_fail(r, e, "features launch");
+    e = launch_features(fa, s);
+    if (e != hipSuccess) return dev_fail(r, e, "features launch");
+    r->prepared = *info;
+    r->prepared_valid = true;
+    return ISTI_OK;
 }
 
-isti_status isti_learned_loss(const float *out, const uint32_t *q_target, uint32_t width, uint32_t height,
-                              uint32_t net_w, uint32_t net_h, float conf_weight, float *dout, float *loss_acc,
-                              void *stream) {
+isti_status isti_learned_loss(isti_renderer *r, const float *out, const uint32_t *q_target, uint32_t net_w,
+                              uint32_t net_h, float conf_weight, float *dout, float *loss_acc, void *stream) {
+    if (!r) return ISTI_ERR_INVALID_ARG;
+    if (!out || !q_target || !dout || !loss_acc) return fail(r, ISTI_ERR_INVALID_ARG, "bad loss arguments");
+    if (!r->prepared_valid) return fail(r, ISTI_ERR_INVALID_ARG, "call isti_learned_features for this frame first");
+    const uint32_t w = r->prepared.width, h = r->prepared.height;
     uint32_t shift = 0;
-    if (!out || !q_target || !dout || !loss_acc || !net_shift_for(width, height, net_w, net_h, shift))
-        return ISTI_ERR_INVALID_ARG;
-    const hipError_t e = launch_learned_loss(out, q_target, width, height, net_w, net_h, shift, conf_weight, dout,
-                                             loss_acc, reinterpret_cast<hipStream_t>(stream));
-    return e == hipSuccess ? ISTI_OK : ISTI_ERR_DEVICE;
+    if (!net_shift_for(w, h, net_w, net_h, shift))
+        return fail(r, ISTI_ERR_UNSUPPORTED_SHAPE, "network grid does not match the render size");
+    const hipError_t e = launch_learned_loss(out, q_target, r->scratch[0][0].cam_ff, r->scratch[1][0].cam_ff, w, h,
+                                             net_w, net_h, shift, conf_weight, dout, loss_acc,
+                                             reinterpret_cast<hipStream_t>(stream));
+    return e == hipSuccess ? ISTI_OK : dev_fail(r, e, "loss launch");
 }
 
 isti_status isti_render_learned(isti_renderer *r, const isti_render_info *info, const float *net_out,
@@ -622,12 +678,12 @@ isti_status isti_render_learned(isti_renderer *r, const isti_render_info *info, 
                                 uint8_t *out, size_t out_pitch, void *stream) {
     if (!r) return ISTI_ERR_INVALID_ARG;
     if (!info || !out || !net_out) return fail(r, ISTI_ERR_INVALID_ARG, "info, out and net_out must be non-null");
-    if (!(conf_threshold > 0.0f && conf_threshold < 1.0f))
-        return fail(r, ISTI_ERR_INVALID_ARG, "conf_threshold must be in (0, 1)");
+    if (!(conf_threshold > 0.0f && conf_threshold <= 1.0f))
+        return fail(r, ISTI_ERR_INVALID_ARG, "conf_threshold must be in (0, 1]");
     isti_status st = validate(r, info, out_pitch);
     if (st != ISTI_OK) return st;
-    if (info->mode == ISTI_MODE_TIMEWARP || info->mode == ISTI_MODE_BACKWARD_BIDIRECTIONAL)
-        return fail(r, ISTI_ERR_INVALID_ARG, "learned starts apply to modes 0, 2 and 3");
+    if (!r->prepared_valid || std::memcmp(&r->prepared, info, sizeof(*info)) != 0)
+        return fail(r, ISTI_ERR_INVALID_ARG, "call isti_learned_features for this frame first");
     hipError_t e = hipSetDevice(r->device);
     if (e != hipSuccess) return dev_fail(r, e, "hipSetDevice");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
@@ -639,32 +695,11 @@ isti_status isti_render_learned(isti_renderer *r, const isti_render_info *info, 
     P.ra.net_out = net_out;
     P.ra.net_w = net_w;
     P.ra.net_h = net_h;
-    P.ra.conf_logit = std::log(conf_threshold / (1.0f - conf_threshold));
+    // confidence threshold as a logit; 1 disables every learned start
+    P.ra.conf_logit = conf_threshold < 1.0f ? std::log(conf_threshold / (1.0f - conf_threshold))
+                                            : std::numeric_limits<float>::infinity();
     P.ra.learned_stats = stats;
-    ClearArgs ca;
-    std::memset(&ca, 0, sizeof(ca));
-    SplatArgs sa;
-    std::memset(&sa, 0, sizeof(sa));
-    sa.vp_cam = P.vp_cam;
-    sa.vp_main[0] = P.vp_main[0];
-    sa.vp_main[1] = P.vp_main[1];
-    sa.init_linear = P.init_linear;
-    uint32_t cb = 0, sb = 0;
-    for (int j = 0; j < P.nj; j++) {
-        sa.job[j] = P.jobs[j];
-        ca.buf[j] = P.jobs[j].splat;
-        ca.n[j] = P.jobs[j].n_px;
-        ca.block_begin[j] = cb;
-        cb += (P.jobs[j].n_px + 511u) / 512u;
-        sa.job[j].block_begin = sb;
-        sb += (P.jobs[j].n_px + SPLAT_BLOCK - 1) / SPLAT_BLOCK;
-    }
-    ca.n_bufs = sa.n_jobs = P.nj;
-    ca.n_blocks = cb;
-    sa.n_blocks = sb;
-    if ((e = mark(r, 0, s)) == hipSuccess) e = launch_clear(ca, s);
-    if (e == hipSuccess && (e = mark(r, 1, s)) == hipSuccess) e = launch_splat(sa, s);
-    if (e == hipSuccess && (e = mark(r, 2, s)) == hipSuccess) e = launch_resolve_learned(P.ra, s);
+    if ((e = mark(r, 2, s)) == hipSuccess) e = launch_resolve_learned(P.ra, s);
     if (e == hipSuccess) e = mark(r, 3, s);
     return e == hipSuccess ? ISTI_OK : dev_fail(r, e, "learned render launch");
 }
@@ -722,6 +757,7 @@ isti_status isti_destroy(isti_renderer *r) {
 
 isti_status isti_keyframe_set(isti_renderer *r, int32_t slot, int32_t view, const isti_view_desc *d) {
     if (!r) return ISTI_ERR_INVALID_ARG;
+    r->prepared_valid = false;
     if (!d || slot < 0 || slot > 1 || view < 0 || view > r->n_aux)
         return fail(r, ISTI_ERR_INVALID_ARG, "slot must be 0/1 and view 0..n_aux, desc non-null");
     if (d->width == 0 || d->height == 0 || d->width > ISTI_MAX_DIM || d->height > ISTI_MAX_DIM)
@@ -736,6 +772,7 @@ isti_status isti_keyframe_set(isti_renderer *r, int32_t slot, int32_t view, cons
 
 isti_status isti_keyframe_swap(isti_renderer *r) {
     if (!r) return ISTI_ERR_INVALID_ARG;
+    r->prepared_valid = false;
     for (int v = 0; v < MAX_VIEWS; v++) {
         std::swap(r->views[0][v], r->views[1][v]);
         std::swap(r->view_set[0][v], r->view_set[1][v]);
@@ -758,38 +795,9 @@ isti_status isti_render(isti_renderer *r, const isti_render_info *info, uint8_t 
     FramePlan P;
     st = plan_frame(r, info, out, out_pitch, P);
     if (st != ISTI_OK) return st;
-
-    ClearArgs ca;
-    std::memset(&ca, 0, sizeof(ca));
-    SplatArgs sa;
-    std::memset(&sa, 0, sizeof(sa));
-    sa.vp_cam = P.vp_cam;
-    sa.vp_main[0] = P.vp_main[0];
-    sa.vp_main[1] = P.vp_main[1];
-    sa.init_linear = P.init_linear;
-    uint32_t clear_blocks = 0, splat_blocks = 0;
-    for (int j = 0; j < P.nj; j++) {
-        SplatJob &J = sa.job[j];
-        J = P.jobs[j];
-        ca.buf[j] = J.splat;
-        ca.n[j] = J.n_px;
-        ca.block_begin[j] = clear_blocks;
-        clear_blocks += (J.n_px + 511u) / 512u;
-        J.block_begin = splat_blocks;
-        splat_blocks += (J.n_px + SPLAT_BLOCK - 1) / SPLAT_BLOCK;
-    }
-    ca.n_bufs = P.nj;
-    ca.n_blocks = clear_blocks;
-    sa.n_jobs = P.nj;
-    sa.n_blocks = splat_blocks;
-
-    if ((e = mark(r, 0, s)) != hipSuccess) return dev_fail(r, e, "event");
-    e = launch_clear(ca, s);
-    if (e != hipSuccess) return dev_fail(r, e, "clear launch");
-    if ((e = mark(r, 1, s)) != hipSuccess) return dev_fail(r, e, "event");
-    e = launch_splat(sa, s);
-    if (e != hipSuccess) return dev_fail(r, e, "splat launch");
-    if ((e = mark(r, 2, s)) != hipSuccess) return dev_fail(r, e, "event");
+    r->prepared_valid = false;
+    st = splat_stage(r, P, s);
+    if (st != ISTI_OK) return st;
     e = launch_resolve(P.ra, P.window, P.always_valid, s);
     if (e != hipSuccess) return dev_fail(r, e, "resolve launch");
     if ((e = mark(r, 3, s)) != hipSuccess) return dev_fail(r, e, "event");

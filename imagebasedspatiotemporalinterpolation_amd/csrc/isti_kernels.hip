@@ -445,7 +445,12 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
                 const ResolveView &v = d ? a.bwd[0] : a.fwd[0];
                 const float *o = a.net_out + (((size_t)d * a.net_h + (y >> a.net_shift)) * a.net_w + (x >> a.net_shift)) * 16;
                 if (o[2] >= a.conf_logit) {
-                    float fx = (float)x + o[0] + 0.5f, fy = (float)y + o[1] + 0.5f;
+                    // one analytic search step from the pixel itself (p - cam_ff(p)),
+                    // corrected by the network's residual in cells of 2^net_shift pixels
+                    const float cell = (float)(1u << a.net_shift);
+                    const float4 m = at(v.cam_ff, v.w, x, y);
+                    float fx = ((float)x + 0.5f - m.x * (0.5f * (float)v.w)) + o[0] * cell;
+                    float fy = ((float)y + 0.5f - m.y * (0.5f * (float)v.h)) + o[1] * cell;
                     fx = fx >= 0.0f ? (fx <= (float)(v.w - 1) ? fx : (float)(v.w - 1)) : 0.0f;  // NaN -> 0
                     fy = fy >= 0.0f ? (fy <= (float)(v.h - 1) ? fy : (float)(v.h - 1)) : 0.0f;
                     si[0][d].x = (uint32_t)fx;
@@ -458,11 +463,13 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
         chain_start(ch[c][0], ran[c][0], si[c][0].x, si[c][0].y, a.fwd[c]);
         chain_start(ch[c][1], ran[c][1], si[c][1].x, si[c][1].y, a.bwd[c]);
     }
+    uint32_t steps = 0;  // main-camera search steps taken (learned statistics)
     for (int i = 0; i < a.max_iter; i++) {
         bool any = false;
 #pragma unroll
         for (int c = 0; c < NC; c++) any = any || ch[c][0].active || ch[c][1].active;
         if (!any) break;
+        if (LEARNED) steps += (uint32_t)ch[0][0].active + (uint32_t)ch[0][1].active;
 #pragma unroll
         for (int c = 0; c < NC; c++) {
             chain_step(ch[c][0], i, a.max_iter, pcs, a.fwd[c]);
@@ -491,12 +498,16 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
         }
     }
     if (LEARNED) {
-        // the plain search judges the network: a learned start whose search
-        // does not end below EPS_1 is redone from the window start
+        // the plain search judges the network: a learned start is redone from
+        // the window start unless its search ends below EPS_1 on a surface
+        // not behind the window's nearest splat candidate (the splat's depth
+        // test; a fixed point on an occluded surface is valid but hidden)
         bool redo[2];
 #pragma unroll
         for (int d = 0; d < 2; d++) {
-            redo[d] = learned[d] && !(err[0][d] < EPS_1_FULL_RES);
+            const bool hidden = window0[d].depth > 0.0f &&
+                                !(depth_to_z(si[0][d].depth) - depth_to_z(window0[d].depth) < EPS_2_FULL_RES);
+            redo[d] = learned[d] && !(err[0][d] < EPS_1_FULL_RES && !hidden);
             if (redo[d]) {
                 si[0][d] = window0[d];
                 ran[0][d] = window0[d].depth > 0.0f && (d == 1 || fc.alpha < 1.0f);
@@ -505,6 +516,7 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
         }
         for (int i = 0; i < a.max_iter; i++) {
             if (!(ch[0][0].active || ch[0][1].active)) break;
+            steps += (uint32_t)ch[0][0].active + (uint32_t)ch[0][1].active;
             chain_step(ch[0][0], i, a.max_iter, pcs, a.fwd[0]);
             chain_step(ch[0][1], i, a.max_iter, pcs, a.bwd[0]);
         }
@@ -531,12 +543,16 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
             const uint32_t rej = (uint32_t)(__popcll(__ballot(redo[0])) + __popcll(__ballot(redo[1])));
             const uint32_t low = (uint32_t)(__popcll(__ballot(fwd_ok && !learned[0])) + __popcll(__ballot(!learned[1])));
             const uint32_t tot = (uint32_t)(__popcll(__ballot(fwd_ok)) + __popcll(__ballot(1)));
+            uint32_t wave_steps = 0;  // sum over the active lanes, one ballot per bit
+#pragma unroll
+            for (int b = 0; b < 16; b++) wave_steps += (uint32_t)__popcll(__ballot((steps >> b) & 1u)) << b;
             const unsigned long long act = __ballot(1);
             if ((int)(threadIdx.x & 63) == __ffsll((long long)act) - 1) {
                 atomicAdd(&a.learned_stats[0], kept);
                 atomicAdd(&a.learned_stats[1], rej);
                 atomicAdd(&a.learned_stats[2], low);
                 atomicAdd(&a.learned_stats[3], tot);
+                atomicAdd(&a.learned_stats[4], wave_steps);
             }
         }
     }
@@ -639,7 +655,10 @@ __global__ __launch_bounds__(256) void features_kernel(const FeatureArgs a) {
         const uint32_t p = y * a.w + x;
         const uchar4 co = own.img[p], cq = oth.img[p];
         const float4 mo = own.cam_ff[p], mq = oth.cam_ff[p];
-        const float hw = 0.5f * (float)a.w, hh = 0.5f * (float)a.h;
+        // displacements in network cells (2^net_shift render pixels), so a
+        // network sees the same numbers for the same scene at any render size
+        const float cell = 1.0f / (float)(1u << a.net_shift);
+        const float hw = 0.5f * (float)a.w * cell, hh = 0.5f * (float)a.h * cell;
         f[0] = co.x / 255.0f; f[1] = co.y / 255.0f; f[2] = co.z / 255.0f;
         f[3] = cq.x / 255.0f; f[4] = cq.y / 255.0f; f[5] = cq.z / 255.0f;
         f[6] = own.z[p]; f[7] = oth.z[p];
@@ -659,9 +678,10 @@ __global__ __launch_bounds__(256) void features_kernel(const FeatureArgs a) {
 
 // Training loss against captured plain-search results (see isti_learned.h).
 __global__ __launch_bounds__(256) void learned_loss_kernel(const float *__restrict__ out, const uint32_t *__restrict__ q,
-                                                           uint32_t w, uint32_t h, uint32_t net_w, uint32_t net_h,
-                                                           uint32_t shift, float conf_weight, float *dout,
-                                                           float *loss_acc) {
+                                                           const float4 *__restrict__ ff_fwd,
+                                                           const float4 *__restrict__ ff_bwd, uint32_t w, uint32_t h,
+                                                           uint32_t net_w, uint32_t net_h, uint32_t shift,
+                                                           float conf_weight, float *dout, float *loss_acc) {
     __shared__ float red[256];
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     const uint32_t cells = net_w * net_h;
@@ -676,17 +696,24 @@ __global__ __launch_bounds__(256) void learned_loss_kernel(const float *__restri
         for (int k = 0; k < 16; k++) g[k] = 0.0f;
         if (x < w && y < h) {
             const float norm = 1.0f / (2.0f * (float)cells);
+            const float cell = 1.0f / (float)(1u << shift);
             const uint32_t t = q[2 * (y * w + x) + d];
             const bool valid = t != 0xffffffffu;
+            bool close = false;  // confidence target: a valid target within one cell
             if (valid) {
-                const float tx = (float)(t & 0xFFFFu) - (float)x, ty = (float)(t >> 16) - (float)y;
+                // target residual in cells: the start (x + 0.5 - m.x w/2) + o cell
+                // lands on the centre of the found pixel
+                const float4 m = (d ? ff_bwd : ff_fwd)[y * w + x];
+                const float tx = ((float)(t & 0xFFFFu) - (float)x + m.x * (0.5f * (float)w)) * cell;
+                const float ty = ((float)(t >> 16) - (float)y + m.y * (0.5f * (float)h)) * cell;
                 const float ex = o[0] - tx, ey = o[1] - ty;
                 const float ax = fabsf(ex), ay = fabsf(ey);
                 loss += (ax < 1.0f ? 0.5f * ex * ex : ax - 0.5f) + (ay < 1.0f ? 0.5f * ey * ey : ay - 0.5f);
                 g[0] = norm * fmaxf(-1.0f, fminf(1.0f, ex));
                 g[1] = norm * fmaxf(-1.0f, fminf(1.0f, ey));
+                close = ax < 1.0f && ay < 1.0f;
             }
-            const float z = o[2], c01 = valid ? 1.0f : 0.0f;
+            const float z = o[2], c01 = close ? 1.0f : 0.0f;
             const float sp = z > 0.0f ? z + log1pf(expf(-z)) : log1pf(expf(z));  // softplus
             loss += conf_weight * (sp - c01 * z);
             g[2] = norm * conf_weight * (1.0f / (1.0f + expf(-z)) - c01);
@@ -913,12 +940,12 @@ hipError_t launch_features(const FeatureArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_learned_loss(const float *out, const uint32_t *q, uint32_t w, uint32_t h, uint32_t net_w,
-                               uint32_t net_h, uint32_t shift, float conf_weight, float *dout, float *loss_acc,
-                               hipStream_t s) {
+hipError_t launch_learned_loss(const float *out, const uint32_t *q, const float4 *ff_fwd, const float4 *ff_bwd,
+                               uint32_t w, uint32_t h, uint32_t net_w, uint32_t net_h, uint32_t shift,
+                               float conf_weight, float *dout, float *loss_acc, hipStream_t s) {
     const uint32_t n = 2u * net_w * net_h;
-    hipLaunchKernelGGL(learned_loss_kernel, dim3((n + 255u) / 256u), dim3(256), 0, s, out, q, w, h, net_w, net_h, shift,
-                       conf_weight, dout, loss_acc);
+    hipLaunchKernelGGL(learned_loss_kernel, dim3((n + 255u) / 256u), dim3(256), 0, s, out, q, ff_fwd, ff_bwd, w, h,
+                       net_w, net_h, shift, conf_weight, dout, loss_acc);
     return hipGetLastError();
 }
 

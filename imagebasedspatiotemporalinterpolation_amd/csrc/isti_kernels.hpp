@@ -144,9 +144,9 @@ struct FeatureArgs {
     float alpha;
 };
 hipError_t launch_features(const FeatureArgs &a, hipStream_t s);
-hipError_t launch_learned_loss(const float *out, const uint32_t *q, uint32_t w, uint32_t h, uint32_t net_w,
-                               uint32_t net_h, uint32_t shift, float conf_weight, float *dout, float *loss_acc,
-                               hipStream_t s);
+hipError_t launch_learned_loss(const float *out, const uint32_t *q, const float4 *ff_fwd, const float4 *ff_bwd,
+                               uint32_t w, uint32_t h, uint32_t net_w, uint32_t net_h, uint32_t shift,
+                               float conf_weight, float *dout, float *loss_acc, hipStream_t s);
 hipError_t launch_resolve_learned(const ResolveArgs &a, hipStream_t s);
 
 hipError_t launch_clear(const ClearArgs &a, hipStream_t s);

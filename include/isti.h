@@ -50,6 +50,8 @@ extern "C" {
 #define ISTI_MAX_DIM 65535
 /* Largest shadow_frames_per_anchor: 2-bit codes in a 24-bit field. */
 #define ISTI_MAX_SHADOW_FRAMES 12
+/* Upper bound of isti_render_info.max_search_iterations. */
+#define ISTI_MAX_SEARCH_ITERATIONS 4096
 
 typedef enum isti_status {
     ISTI_OK = 0,
@@ -102,8 +104,9 @@ typedef struct isti_view_desc {
 /* RenderInfo + AdditionalRenderInfo (include/renderer.h:18-35).
  * max_search_iterations extends the reference: < 0 selects the mode's own
  * count (modes 0/2: 3, mode 1: 5, mode 3: 0, src/renderer.cu:678-689);
- * a value >= 0 caps the fixed-point search at that many steps (it stops
- * early at a fixed point, which is bit-identical to running all steps). */
+ * a value in [0, ISTI_MAX_SEARCH_ITERATIONS] caps the fixed-point search at
+ * that many steps (it stops early at a fixed point, which is bit-identical
+ * to running all steps). */
 typedef struct isti_render_info {
     uint32_t width;
     uint32_t height;

@@ -71,30 +71,45 @@ isti_status isti_unet_adam(isti_unet *net, float lr, float beta1, float beta2, f
  * its reprojection error is below EPS_1, else 0xffffffff. NULL disables. */
 isti_status isti_set_search_capture(isti_renderer *r, uint32_t *q_out);
 
-/* Builds the network input of both directions (batch 2: fwd, bwd) at
- * net_w x net_h (render size / 2^k, multiples of 8 after padding) from the
- * renderer's keyframes and the cam_ff planes of its last render with `info`.
- * x: device bf16 [2][net_h][net_w][16]. */
+/* First half of a learned-start render: runs the frame's splat stage (M1
+ * clear and K1 reprojection + splat of every view, as isti_render does) and
+ * builds the network input of both directions (batch 2: fwd, bwd) at
+ * net_w x net_h (render size / 2^k for k in 0..3, each rounded up to a
+ * multiple of 8) from the main keyframes and the fresh cam_ff planes.
+ * x: device bf16 [2][net_h][net_w][16]. Displacement features are in
+ * network cells (2^k render pixels). */
 isti_status isti_learned_features(isti_renderer *r, const isti_render_info *info, uint16_t *x, uint32_t net_w,
                                   uint32_t net_h, void *stream);
 
-/* Training loss of a network output against captured plain-search results
- * (q_target as written by isti_set_search_capture, render size W x H):
- * Huber (delta 1 px) on the flow offset of pixels with a valid target, plus
- * binary cross-entropy of the confidence against "the target is valid".
- * Writes dout (same shape as out) and adds the mean loss to *loss_acc
- * (device float). */
-isti_status isti_learned_loss(const float *out, const uint32_t *q_target, uint32_t width, uint32_t height,
-                              uint32_t net_w, uint32_t net_h, float conf_weight, float *dout, float *loss_acc,
-                              void *stream);
+/* Training loss of a network output for the frame prepared by the last
+ * isti_learned_features call, against its captured plain-search results
+ * (q_target as written by isti_set_search_capture): Huber (delta 1 cell) on
+ * the start residual of pixels with a valid target (the residual that puts
+ * the learned start, see isti_render_learned, on the centre of the found
+ * pixel), plus conf_weight times the binary cross-entropy of the confidence
+ * logit against "the target is valid and the predicted residual is within
+ * one cell of it" (the network learns where its own start is right). Each
+ * cell is scored at its top-left render pixel. Writes dout (same shape as
+ * out) and adds the mean loss to *loss_acc (device float). */
+isti_status isti_learned_loss(isti_renderer *r, const float *out, const uint32_t *q_target, uint32_t net_w,
+                              uint32_t net_h, float conf_weight, float *dout, float *loss_acc, void *stream);
 
-/* Renderer::render with learned search starts for the main camera: where
- * the network's confidence is >= conf_threshold the fwd/bwd searches start
- * at the predicted pixel; a start whose search ends with an error >= EPS_1,
- * and every low-confidence pixel, uses the reference's window start instead.
- * net_out: fp32 [2][net_h][net_w][16] for this frame. If stats is non-null it
- * receives (device, uint32[4], accumulated): learned starts kept, learned
- * starts rejected (fallback), low-confidence pixels, total searches. */
+/* Second half: the resolve of the frame prepared by the last
+ * isti_learned_features call (same info, keyframes unchanged since), with
+ * learned starts for the main camera. Where the network's confidence is
+ * >= conf_threshold, the fwd/bwd search of pixel (x, y) starts at
+ * floor((x + 0.5 - m.x * W/2) + o.x * 2^k), likewise in y, clamped to the
+ * view: one analytic search step from the pixel itself (m = the direction's
+ * cam_ff at the pixel, in NDC) corrected by the network's residual o
+ * (channels 0/1, in cells of 2^k pixels). A learned result is kept when its
+ * search ends with an error below EPS_1 on a surface not behind the window
+ * start's candidate (depth_to_z difference < EPS_2, the splat's depth test);
+ * a rejected learned start, and every low-confidence pixel, uses the
+ * reference's window start instead. conf_threshold 1 disables the learned starts (the
+ * plain resolve, with statistics). net_out: fp32 [2][net_h][net_w][16]. If
+ * stats is non-null it receives (device, uint32[5], accumulated): learned
+ * starts kept, learned starts rejected (fallback), low-confidence searches,
+ * total searches, main-camera search steps taken. */
 isti_status isti_render_learned(isti_renderer *r, const isti_render_info *info, const float *net_out,
                                 uint32_t net_w, uint32_t net_h, float conf_threshold, uint32_t *stats,
                                 uint8_t *out_rgba8, size_t out_pitch_bytes, void *stream);

@@ -755,6 +755,39 @@ extern "C" void oracle_search(const float *cam_ff, uint32_t low_w, uint32_t low_
     *qx = r.x;
     *qy = r.y;
 }
+/* Whole-render main-camera searches (checks the learned starts): for every
+ * target pixel p of a w x h render whose keyframe view has the render
+ * resolution, the search (src/renderer.cu:177-202) from start[p] (packed
+ * y << 16 | x; 0xffffffff = no search) and its reprojection error
+ * (src/renderer.cu:204-215; FLT_MAX when no search ran). */
+extern "C" void oracle_search_field(const float *cam_ff, uint32_t w, uint32_t h, const uint32_t *start, int iterations,
+                                    uint32_t *q, float *err) {
+    const F4 *ff = reinterpret_cast<const F4 *>(cam_ff);
+    for (uint32_t y = 0; y < h; y++)
+        for (uint32_t x = 0; x < w; x++) {
+            const size_t i = (size_t)y * w + x;
+            if (start[i] == 0xffffffffu) {
+                q[i] = 0xffffffffu;
+                err[i] = FLT_MAX;
+                continue;
+            }
+            const U2 r = search(U2{start[i] & 0xFFFFu, start[i] >> 16}, U2{x, y}, U2{w, h}, U2{w, h}, ff, iterations);
+            err[i] = reproj_error(r, U2{w, h}, pixel2cs(U2{x, y}, U2{w, h}), ff[(size_t)r.y * w + r.x]);
+            q[i] = r.y << 16 | r.x;
+        }
+}
+/* The window starts (src/renderer.cu:148-175) of every pixel of a w x h
+ * splat buffer, packed as above (no candidate, depth <= 0: 0xffffffff), and,
+ * if depth is non-null, the candidate's decoded depth. */
+extern "C" void oracle_window_starts(const uint64_t *splat, uint32_t w, uint32_t h, int window, uint32_t *start,
+                                     float *depth) {
+    for (uint32_t y = 0; y < h; y++)
+        for (uint32_t x = 0; x < w; x++) {
+            const SplatInfo s = init_search(U2{x, y}, U2{w, h}, splat, window);
+            start[(size_t)y * w + x] = s.depth > 0.0f ? (s.src.y << 16 | s.src.x) : 0xffffffffu;
+            if (depth) depth[(size_t)y * w + x] = s.depth;
+        }
+}
 extern "C" void oracle_color_from_shadow(const isti_view_desc *d, int forward, uint32_t idx_x, uint32_t idx_y,
                                          float t, int sfpa, int no_shadows, uint32_t px, uint32_t py,
                                          uint8_t out[4]) {

@@ -71,6 +71,14 @@ void oracle_init_search(const uint64_t *splat, uint32_t w, uint32_t h, uint32_t 
 void oracle_search(const float *cam_ff, uint32_t low_w, uint32_t low_h, uint32_t start_x,
                    uint32_t start_y, uint32_t px, uint32_t py, uint32_t full_w, uint32_t full_h,
                    int iterations, uint32_t *qx, uint32_t *qy);
+/* Every main-camera search of a w x h render from packed starts
+ * (y << 16 | x, 0xffffffff = none) with its reprojection error. */
+void oracle_search_field(const float *cam_ff, uint32_t w, uint32_t h, const uint32_t *start, int iterations,
+                         uint32_t *q, float *err);
+/* Window starts of every pixel of a splat buffer, packed as above, and
+ * (if depth is non-null) the candidates' decoded depths. */
+void oracle_window_starts(const uint64_t *splat, uint32_t w, uint32_t h, int window, uint32_t *start,
+                          float *depth);
 /* colorFromShadow(idcs, t, ...) (src/renderer.cu:270-295) for one pixel of
  * a view whose planes are host pointers. */
 void oracle_color_from_shadow(const isti_view_desc *view, int forward, uint32_t idx_x,

@@ -55,6 +55,8 @@ def lib() -> ctypes.CDLL:
         L.oracle_search.argtypes = [c_void_p] + [c_uint32] * 8 + [c_int, POINTER(c_uint32), POINTER(c_uint32)]
         L.oracle_color_from_shadow.argtypes = [POINTER(ViewDesc), c_int, c_uint32, c_uint32, c_float, c_int, c_int,
                                                c_uint32, c_uint32, c_void_p]
+        L.oracle_search_field.argtypes = [c_void_p, c_uint32, c_uint32, c_void_p, c_int, c_void_p, c_void_p]
+        L.oracle_window_starts.argtypes = [c_void_p, c_uint32, c_uint32, c_int, c_void_p, c_void_p]
         L.oracle_mean_z.argtypes = [c_void_p, c_uint32, c_uint32]
         L.oracle_mean_z.restype = c_float
         _lib = L
@@ -135,3 +137,25 @@ def render(fwd: list, bwd: list, info: RenderInfoC, out: np.ndarray, priorities=
 def mean_z(z: np.ndarray) -> float:
     z = np.ascontiguousarray(z, np.float32)
     return float(lib().oracle_mean_z(z.ctypes.data, z.shape[1], z.shape[0]))
+
+
+def window_starts(splat: np.ndarray, window: int = 1, with_depth: bool = False):
+    """Packed (y << 16 | x) window starts of a splat buffer [H, W] u64
+    (and their decoded depths if with_depth)."""
+    splat = np.ascontiguousarray(splat, np.uint64)
+    h, w = splat.shape
+    out = np.zeros((h, w), np.uint32)
+    depth = np.zeros((h, w), np.float32)
+    lib().oracle_window_starts(splat.ctypes.data, w, h, window, out.ctypes.data, depth.ctypes.data)
+    return (out, depth) if with_depth else out
+
+
+def search_field(cam_ff: np.ndarray, start: np.ndarray, iterations: int):
+    """(q packed [H, W] u32, error [H, W] f32) of the searches from `start`."""
+    cam_ff = np.ascontiguousarray(cam_ff, np.float32)
+    start = np.ascontiguousarray(start, np.uint32)
+    h, w = start.shape
+    q = np.zeros((h, w), np.uint32)
+    err = np.zeros((h, w), np.float32)
+    lib().oracle_search_field(cam_ff.ctypes.data, w, h, start.ctypes.data, iterations, q.ctypes.data, err.ctypes.data)
+    return q, err

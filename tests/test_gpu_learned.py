@@ -1,0 +1,189 @@
+"""Learned search starts on the GPU: the learned resolve against the oracle's
+searches from the same starts (with the plain-search fallback), the disabled
+path against the plain render, training and checkpoints, and the trained
+network against the plain search."""
+import numpy as np
+import pytest
+import torch
+
+from imagebasedspatiotemporalinterpolation_amd import learned as Ln
+from imagebasedspatiotemporalinterpolation_amd import renderer as R
+from tests import _frames as F
+
+pytestmark = pytest.mark.gpu
+
+EPS_1 = np.float32(0.002)  # EPS_1_FULL_RES, src/renderer.cu:16
+EPS_2 = np.float32(0.001)  # EPS_2_FULL_RES, src/renderer.cu:18
+NONE = np.uint32(0xFFFFFFFF)
+F1, HALF, TWO = np.float32(1), np.float32(0.5), np.float32(2)
+
+
+def _frame(seed=11, div=2, b=2):
+    cfg = F.scaled(1, seed, div)
+    fwd, bwd = F.pair(cfg, 0)
+    vp, ivp, w, h = F.target(cfg, 0, b)
+    return cfg, fwd, bwd, vp, ivp, w, h, b / cfg.keyframe_spacing
+
+
+def _gpu_setup(fwd, bwd, vp, ivp, w, h, alpha, fpa, max_iter):
+    r = R.Renderer(0)
+    fd = R.FrameData(R.KeyframeView.from_numpy(fwd[0]), R.KeyframeView.from_numpy(bwd[0]))
+    r.set_frame(fd)
+    info = R.RenderInfo((w, h), alpha, vp, ivp, R.AdditionalRenderInfo(fpa, 0, True, R.RenderMode.OURS), max_iter)
+    return r, fd, info
+
+
+def _expected_capture(O, sc, zs, starts_learned, learned_mask, iters, alpha):
+    """Per direction: the oracle's search result from the learned start where
+    it is kept (error < EPS_1 and not behind the window candidate's surface
+    by EPS_2 or more), else the plain search's; packed, or NONE."""
+    out = []
+    for d in (0, 1):
+        cam, splat = sc[(d, 0)]["cam_ff"], sc[(d, 0)]["splat"]
+        ws, wdepth = O.window_starts(splat, 1, with_depth=True)
+        if d == 0 and not alpha < 1.0:
+            ws[:] = NONE
+        qp, ep = O.search_field(cam, ws, iters)
+        plain = np.where(ep < EPS_1, qp, NONE)
+        ls = np.where(learned_mask[d], starts_learned[d], NONE)
+        ql, el = O.search_field(cam, ls, iters)
+        qx, qy = (ql & 0xFFFF).astype(np.int64), (ql >> 16).astype(np.int64)
+        ok = ql != NONE
+        qx, qy = np.where(ok, qx, 0), np.where(ok, qy, 0)
+        ldepth = (zs[d][qy, qx] + cam[qy, qx, 2] + F1) * HALF  # z_to_depth(z[q] + cam_ff[q].z)
+        hidden = (wdepth > 0) & ~((ldepth * TWO - F1) - (wdepth * TWO - F1) < EPS_2)
+        kept = learned_mask[d] & (el < EPS_1) & ~hidden
+        out.append((np.where(kept, ql, plain), kept, learned_mask[d] & ~kept))
+    return out
+
+
+@pytest.mark.parametrize("shift,iters", [(0, 3), (1, 32), (2, 5)])
+def test_learned_starts_match_oracle(oracle_lib, shift, iters):
+    O = oracle_lib
+    cfg, fwd, bwd, vp, ivp, w, h, alpha = _frame()
+    fwd, bwd = fwd[:1], bwd[:1]
+    _, sc = F.oracle_frame(O, fwd, bwd, vp, ivp, alpha, cfg.keyframe_spacing, 0, max_iter=iters, no_shadows=True)
+    r, fd, info = _gpu_setup(fwd, bwd, vp, ivp, w, h, alpha, cfg.keyframe_spacing, iters)
+    nw, nh = Ln.net_grid(w, h, shift)
+    # crafted network output: offsets in cells from a seeded pattern (exact in
+    # float32), confident on about 2/3 of the cells
+    rng = np.random.default_rng(shift)
+    net = np.zeros((2, nh, nw, 16), np.float32)
+    net[..., 0] = rng.integers(-6, 7, (2, nh, nw)) * 0.5
+    net[..., 1] = rng.integers(-6, 7, (2, nh, nw)) * 0.5
+    net[..., 2] = np.where(rng.random((2, nh, nw)) < 0.67, 3.0, -3.0)
+    x = torch.zeros((2, nh, nw, 16), dtype=torch.int16, device="cuda")
+    q = torch.zeros((h, w, 2), dtype=torch.int32, device="cuda")
+    out = torch.zeros((h, w, 4), dtype=torch.uint8, device="cuda")
+    stats = torch.zeros(8, dtype=torch.int32, device="cuda")
+    Ln.set_search_capture(r, q)
+    Ln.features(r, info, x)
+    Ln.render_learned(r, out, info, torch.from_numpy(net).cuda(), 0.5, stats)
+    torch.cuda.synchronize()
+    Ln.set_search_capture(r, None)
+    got = q.cpu().numpy().view(np.uint32)
+
+    cell = np.float32(1 << shift)
+    yy, xx = np.mgrid[0:h, 0:w]
+    starts, masks = [], []
+    for d in (0, 1):
+        o = net[d][yy >> shift, xx >> shift]
+        m = sc[(d, 0)]["cam_ff"]
+        # one analytic step from the pixel plus the residual, in float32 as the kernel rounds it
+        fx = (xx.astype(np.float32) + np.float32(0.5) - m[..., 0] * np.float32(0.5 * w)) + o[..., 0] * cell
+        fy = (yy.astype(np.float32) + np.float32(0.5) - m[..., 1] * np.float32(0.5 * h)) + o[..., 1] * cell
+        sx = np.where(np.isnan(fx), 0, np.clip(fx, 0, w - 1)).astype(np.uint32)
+        sy = np.where(np.isnan(fy), 0, np.clip(fy, 0, h - 1)).astype(np.uint32)
+        starts.append((sy << np.uint32(16)) | sx)
+        m = o[..., 2] >= 0.0
+        if d == 0 and not alpha < 1.0:
+            m[:] = False
+        masks.append(m)
+    exp = _expected_capture(O, sc, (fwd[0]["z"], bwd[0]["z"]), starts, masks, iters, alpha)
+    for d in (0, 1):
+        bad = np.argwhere(got[..., d] != exp[d][0])
+        assert bad.size == 0, f"direction {d}: {len(bad)} pixels differ, first {bad[:5].tolist()}"
+    st = stats.cpu().numpy()
+    kept = sum(int(e[1].sum()) for e in exp)
+    rej = sum(int(e[2].sum()) for e in exp)
+    searches = (2 if alpha < 1.0 else 1) * w * h
+    assert st[0] == kept and st[1] == rej, (st[:5], kept, rej)
+    assert st[0] + st[1] + st[2] == st[3] == searches, st[:5]
+    assert kept > 0 and rej > 0
+    r.close()
+
+
+def test_learned_disabled_equals_plain_render(oracle_lib):
+    """conf_threshold 1: every search starts in the window, so the learned
+    resolve is the plain one, pixel for pixel, and counts its steps."""
+    cfg, fwd, bwd, vp, ivp, w, h, alpha = _frame(seed=4, div=1, b=1)
+    fwd, bwd = fwd[:1], bwd[:1]
+    r, fd, info = _gpu_setup(fwd, bwd, vp, ivp, w, h, alpha, cfg.keyframe_spacing, 32)
+    plain = torch.zeros((h, w, 4), dtype=torch.uint8, device="cuda")
+    r.render(plain, info)
+    li = Ln.LearnedInit(w, h, 1)
+    learned = torch.zeros_like(plain)
+    li.render(r, learned, info, conf_threshold=1.0, with_stats=True)
+    torch.cuda.synchronize()
+    assert torch.equal(plain, learned)
+    s = li.read_stats()
+    assert s["kept"] == 0 and s["rejected"] == 0 and s["low_confidence"] == s["searches"] == 2 * w * h
+    assert 0 < s["steps"] <= 32 * s["searches"]
+    li.close()
+    r.close()
+
+
+def test_render_learned_needs_its_features():
+    cfg, fwd, bwd, vp, ivp, w, h, alpha = _frame(seed=2, div=2, b=1)
+    r, fd, info = _gpu_setup(fwd[:1], bwd[:1], vp, ivp, w, h, alpha, cfg.keyframe_spacing, -1)
+    li = Ln.LearnedInit(w, h, 1)
+    buf = torch.zeros((h, w, 4), dtype=torch.uint8, device="cuda")
+    with pytest.raises(Exception, match="isti_learned_features"):
+        Ln.render_learned(r, buf, info, li.out)
+    Ln.features(r, info, li.x)
+    info2 = R.RenderInfo(info.resolution, 0.5, vp, ivp, info.additional_info, -1)  # another alpha
+    with pytest.raises(Exception, match="isti_learned_features"):
+        Ln.render_learned(r, buf, info2, li.out)
+    Ln.render_learned(r, buf, info, li.out)
+    r.swap()  # keyframes changed since the features
+    with pytest.raises(Exception, match="isti_learned_features"):
+        Ln.render_learned(r, buf, info, li.out)
+    torch.cuda.synchronize()
+    li.close()
+    r.close()
+
+
+def test_short_training_lowers_the_loss_and_checkpoint_round_trip(tmp_path):
+    cfg = Ln.TrainConfig(pairs=90, sequences=3, width=320, height=176, lr=2e-3, lr_final=2e-3)
+    params, losses = Ln.train(cfg, log=None)
+    assert np.all(np.isfinite(losses))
+    assert losses[-30:].mean() < 0.8 * losses[:30].mean(), (losses[:30].mean(), losses[-30:].mean())
+    p = tmp_path / "w.bin"
+    Ln.save_params(p, params, {"pairs": cfg.pairs})
+    back, meta = Ln.load_params(p)
+    assert meta == {"pairs": 90} and np.array_equal(back, params)
+
+
+@pytest.mark.skipif(not Ln.WEIGHTS.exists(), reason="no trained weights in the package")
+def test_trained_network_against_plain_search_and_untrained():
+    """The shipped network at 1920x1080 (network grid 480x272) on a held-out
+    scene, against the plain search and against the same network untrained.
+    Training raises the share of searches that keep the learned start and
+    lowers the search steps below the untrained network's and the plain
+    search's; the plain search's judgement keeps the image within 0.1 dB of
+    the plain output's PSNR. The step saving is small by nature: the plain
+    window start already sits at the search's fixed point almost everywhere
+    on the main camera."""
+    params, meta = Ln.load_params(Ln.WEIGHTS)
+    trained = Ln.evaluate(params, 1920, 1080, 2, seeds=(900003,), pairs_per_seed=1)
+    untrained_params = Ln.UNet(8, 8, batch=1, seed=7).get_params()
+    untrained = Ln.evaluate(untrained_params, 1920, 1080, 2, seeds=(900003,), pairs_per_seed=1)
+    print("trained", trained)
+    print("untrained", untrained)
+    assert trained["kept_rate"] > 0.8, trained
+    assert trained["kept_rate"] > untrained["kept_rate"] + 0.05, (trained, untrained)
+    assert trained["steps_per_search_learned"] < untrained["steps_per_search_learned"], (trained, untrained)
+    assert trained["steps_per_search_learned"] < trained["steps_per_search_plain"], trained
+    assert abs(trained["delta_psnr_db"]) <= 0.1, trained
+    assert trained["psnr_learned_vs_plain"] >= 40.0, trained
+    assert abs(untrained["delta_psnr_db"]) <= 0.1, untrained  # the fallback guards quality either way

@@ -168,3 +168,45 @@ def test_mean_z_exact(oracle_lib):
     z = (rng.integers(-512, 512, (37, 129)) / 1024.0).astype(np.float32)
     expect = math.fsum(z.astype(np.float64).ravel()) / z.size
     assert oracle_lib.mean_z(z) == np.float32(expect)
+
+
+def test_search_field_and_window_starts_compose_the_pixel_routines(oracle_lib):
+    """The whole-render helpers (used to check learned starts) agree with the
+    per-pixel init_search / search / reproj_error entry points, and a search
+    started at its own result stays there (fixed point)."""
+    from tests import _frames as F
+
+    O = oracle_lib
+    L = O.lib()
+    cfg = F.scaled(1, 3, 4)
+    fwd, bwd = F.pair(cfg, 0)
+    vp, ivp, w, h = F.target(cfg, 0, 2)
+    _, sc = F.oracle_frame(O, fwd, bwd, vp, ivp, 0.5, 4, 0, max_iter=8, no_shadows=True)
+    cam, splat = sc[(1, 0)]["cam_ff"], sc[(1, 0)]["splat"]
+    ws = O.window_starts(splat, 1)
+    q, err = O.search_field(cam, ws, 8)
+    rng = np.random.default_rng(0)
+    d, sx, sy, qx, qy, pcs = (ctypes.c_float(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32(),
+                              ctypes.c_uint32(), (ctypes.c_float * 2)())
+    cam_c = np.ascontiguousarray(cam, np.float32)
+    for y, x in zip(rng.integers(0, h, 400), rng.integers(0, w, 400)):
+        L.oracle_init_search(splat.ctypes.data, w, h, int(x), int(y), 1, ctypes.byref(d), ctypes.byref(sx),
+                             ctypes.byref(sy))
+        if not d.value > 0:
+            assert ws[y, x] == 0xFFFFFFFF and q[y, x] == 0xFFFFFFFF and err[y, x] == np.finfo(np.float32).max
+            continue
+        assert ws[y, x] == (sy.value << 16 | sx.value)
+        L.oracle_search(cam_c.ctypes.data, w, h, sx.value, sy.value, int(x), int(y), w, h, 8, ctypes.byref(qx),
+                        ctypes.byref(qy))
+        assert q[y, x] == (qy.value << 16 | qx.value)
+        L.oracle_pixel2cs(int(x), int(y), w, h, pcs)
+        m = cam_c[qy.value, qx.value]
+        e = L.oracle_reproj_error(qx.value, qy.value, w, h, pcs[0], pcs[1], float(m[0]), float(m[1]))
+        assert err[y, x] == np.float32(e)
+    # a converged search restarted at its result does not move
+    conv = (q != 0xFFFFFFFF)
+    q2, err2 = O.search_field(cam, np.where(conv, q, np.uint32(0xFFFFFFFF)), 8)
+    q1, _ = O.search_field(cam, np.where(conv, q, np.uint32(0xFFFFFFFF)), 1)
+    fixed = conv & (q1 == q)
+    assert fixed.mean() > 0.5
+    assert np.array_equal(q2[fixed], q[fixed]) and np.array_equal(err2[fixed], err[fixed])
