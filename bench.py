@@ -48,6 +48,7 @@ def parse():
     p.add_argument("--serving-seconds", type=float, default=2.0,
                    help="duration of the 48-session serving mix (0 disables it)")
     p.add_argument("--stereo-frames", type=int, default=60, help="stereo 2x2064x2208 frames timed (0 disables)")
+    p.add_argument("--no-autotune", action="store_true", help="keep the default 16x16 resolve tile")
     p.add_argument("--learned-frames", type=int, default=201,
                    help="1080p frames timed for learned-init vs plain (0 disables)")
     p.add_argument("--copy-probe", action="store_true",
@@ -290,6 +291,17 @@ def main():
         for ci in infos:
             frame(ci)
     torch.cuda.synchronize()
+    # pick the resolve tile shape for this size / aux count / window on this GPU
+    tuned = None
+    if not args.no_autotune:
+        ri_mid = R.RenderInfo((W, H), infos[len(infos) // 2].alpha, np.ctypeslib.as_array(infos[len(infos) // 2].viewproj),
+                              np.ctypeslib.as_array(infos[len(infos) // 2].inv_viewproj),
+                              R.AdditionalRenderInfo(fpa, sfpa, False, R.RenderMode(args.mode)), args.iters)
+        best, tms = rend.autotune(out, ri_mid, reps=20)
+        tuned = {"tile": list(N.TILE_SHAPES[best]),
+                 "resolve_ms": {f"{a}x{b}": round(t, 5) for (a, b), t in zip(N.TILE_SHAPES, tms)}}
+        print(f"[bench] autotuned resolve tile {N.TILE_SHAPES[best]}: {tuned['resolve_ms']}", file=sys.stderr,
+              flush=True)
 
     n_frames = args.steps * len(infos)
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n_frames)]
@@ -345,6 +357,7 @@ def main():
             "max_search_iterations": args.iters,
             "mode": args.mode,
         },
+        "autotune": tuned,
         "p50_frame_ms": round(float(np.percentile(frame_ms, 50)), 4),
         "p99_frame_ms": round(float(np.percentile(frame_ms, 99)), 4),
         "stage_avg_ms": {k2: round(v2, 5) for k2, v2 in stage_avg_ms.items()},

@@ -40,7 +40,11 @@ EXPORTED_SYMBOLS = (
     "isti_batch_set_graph",
     "isti_render_batch",
     "isti_batch_last_error",
+    "isti_autotune",
+    "isti_set_tile_shape",
 )
+ISTI_N_TILE_SHAPES = 4
+TILE_SHAPES = ((16, 16), (32, 8), (64, 4), (8, 32))
 
 
 class ViewDesc(ctypes.Structure):
@@ -154,6 +158,11 @@ def lib() -> ctypes.CDLL:
     L.isti_render_batch.restype = c_int32
     L.isti_batch_last_error.argtypes = [c_void_p]
     L.isti_batch_last_error.restype = c_char_p
+    L.isti_autotune.argtypes = [c_void_p, POINTER(RenderInfoC), c_void_p, c_size_t, c_int32, POINTER(c_float),
+                                POINTER(c_int32), c_void_p]
+    L.isti_autotune.restype = c_int32
+    L.isti_set_tile_shape.argtypes = [c_void_p, c_int32]
+    L.isti_set_tile_shape.restype = c_int32
     _lib = L
     return L
 

@@ -15,6 +15,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -60,12 +61,15 @@ def build_native(verbose: bool = False) -> None:
     """Compile libisti.so (HIP), libisti_scene.so and the splinter driver."""
     obj_dir = PKG / "_build"
     obj_dir.mkdir(exist_ok=True)
-    objs = []
-    for src in ("isti_kernels.hip", "isti_api.cpp", "unet.hip"):
+    cmds, objs = [], []
+    for src in ("resolve_kernels.hip", "isti_kernels.hip", "unet.hip", "isti_api.cpp"):
         obj = obj_dir / (src.rsplit(".", 1)[0] + ".o")
         extra = ["-x", "hip"] if src.endswith(".hip") else []
-        _run([HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", *extra, "-c", CSRC / src, "-o", obj], verbose)
+        cmds.append([HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", *extra, "-c", CSRC / src, "-o", obj])
         objs.append(obj)
+    with ThreadPoolExecutor(max_workers=min(len(cmds), os.cpu_count() or 1)) as pool:
+        for f in [pool.submit(_run, c, verbose) for c in cmds]:
+            f.result()
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp], verbose)
     os.replace(tmp, LIB)

@@ -16,7 +16,9 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <mutex>
 #include <new>
+#include <tuple>
 #include <string>
 #include <utility>
 #include <vector>
@@ -30,6 +32,11 @@ using namespace isti;
 namespace {
 
 thread_local std::string g_create_error;
+
+// Tile shapes chosen by isti_autotune, per (width, height, n_aux, window).
+using TuneKey = std::tuple<uint32_t, uint32_t, int, int>;
+std::mutex g_tuned_mutex;
+std::map<TuneKey, int> g_tuned;
 
 struct Scratch {
     uint32_t w = 0, h = 0;
@@ -68,6 +75,7 @@ struct isti_renderer {
     // frame whose splat stage isti_learned_features ran last (isti_render_learned resolves it)
     isti_render_info prepared{};
     bool prepared_valid = false;
+    int tile_shape = -1;  // forced resolve tile shape, or -1 (tuned / default)
     std::string err;
 };
 
@@ -333,6 +341,20 @@ isti_status plan_frame(isti_renderer *r, const isti_render_info *info, uint8_t *
     ra.n_tiles = ra.tiles_x * ((ra.h + TILE_H - 1) / TILE_H);
     ra.q_out = r->q_out;
     return ISTI_OK;
+}
+
+// Tile grid of a resolve launch for tile shape `shape`.
+void set_tiles(ResolveArgs &ra, int shape) {
+    const uint32_t tw = TILE_SHAPE_W[shape], th = TILE_SHAPE_H[shape];
+    ra.tiles_x = (ra.w + tw - 1) / tw;
+    ra.n_tiles = ra.tiles_x * ((ra.h + th - 1) / th);
+}
+
+int tile_shape_for(const isti_renderer *r, const FramePlan &P) {
+    if (r->tile_shape >= 0) return r->tile_shape;
+    std::lock_guard<std::mutex> lock(g_tuned_mutex);
+    const auto it = g_tuned.find(TuneKey{P.ra.w, P.ra.h, P.ra.n_aux, P.window});
+    return it == g_tuned.end() ? 0 : it->second;
 }
 
 // M1 clear and K1 reprojection + splat of every view of a planned frame
@@ -795,10 +817,12 @@ isti_status isti_render(isti_renderer *r, const isti_render_info *info, uint8_t 
     FramePlan P;
     st = plan_frame(r, info, out, out_pitch, P);
     if (st != ISTI_OK) return st;
+    const int shape = tile_shape_for(r, P);
+    set_tiles(P.ra, shape);
     r->prepared_valid = false;
     st = splat_stage(r, P, s);
     if (st != ISTI_OK) return st;
-    e = launch_resolve(P.ra, P.window, P.always_valid, s);
+    e = launch_resolve(P.ra, P.window, P.always_valid, shape, s);
     if (e != hipSuccess) return dev_fail(r, e, "resolve launch");
     if ((e = mark(r, 3, s)) != hipSuccess) return dev_fail(r, e, "event");
     return ISTI_OK;
@@ -812,6 +836,61 @@ isti_status isti_scratch(const isti_renderer *r, int32_t slot, int32_t view, ist
     out->splat = reinterpret_cast<const uint64_t *>(s.splat);
     out->cam_ff = reinterpret_cast<const float *>(s.cam_ff);
     out->full_ff = reinterpret_cast<const float *>(s.full_ff);
+    return ISTI_OK;
+}
+
+isti_status isti_set_tile_shape(isti_renderer *r, int32_t shape) {
+    if (!r) return ISTI_ERR_INVALID_ARG;
+    if (shape < -1 || shape >= N_TILE_SHAPES) return fail(r, ISTI_ERR_INVALID_ARG, "tile shape must be -1..3");
+    r->tile_shape = shape;
+    return ISTI_OK;
+}
+
+isti_status isti_autotune(isti_renderer *r, const isti_render_info *info, uint8_t *out, size_t out_pitch,
+                          int32_t reps, float *times_ms, int32_t *best, void *stream) {
+    if (!r) return ISTI_ERR_INVALID_ARG;
+    if (!info || !out || reps < 1 || reps > 1000)
+        return fail(r, ISTI_ERR_INVALID_ARG, "info and out must be non-null and reps in 1..1000");
+    isti_status st = validate(r, info, out_pitch);
+    if (st != ISTI_OK) return st;
+    if (info->mode == ISTI_MODE_TIMEWARP) return fail(r, ISTI_ERR_INVALID_ARG, "timewarp has no resolve pass");
+    hipError_t e = hipSetDevice(r->device);
+    if (e != hipSuccess) return dev_fail(r, e, "hipSetDevice");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    FramePlan P;
+    st = plan_frame(r, info, out, out_pitch, P);
+    if (st != ISTI_OK) return st;
+    r->prepared_valid = false;
+    const bool timing = r->timing;
+    r->timing = false;  // the tuning launches stay out of the stage statistics
+    st = splat_stage(r, P, s);
+    r->timing = timing;
+    if (st != ISTI_OK) return st;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    e = hipEventCreate(&ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+    float ms[N_TILE_SHAPES] = {};
+    int pick = 0;
+    for (int k = 0; k < N_TILE_SHAPES && e == hipSuccess; k++) {
+        set_tiles(P.ra, k);
+        e = launch_resolve(P.ra, P.window, P.always_valid, k, s);  // warm-up
+        if (e == hipSuccess) e = hipEventRecord(ev[0], s);
+        for (int i = 0; i < reps && e == hipSuccess; i++) e = launch_resolve(P.ra, P.window, P.always_valid, k, s);
+        if (e == hipSuccess) e = hipEventRecord(ev[1], s);
+        if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms[k], ev[0], ev[1]);
+        ms[k] /= (float)reps;
+        if (ms[k] < ms[pick]) pick = k;
+    }
+    for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
+    if (e != hipSuccess) return dev_fail(r, e, "autotune");
+    {
+        std::lock_guard<std::mutex> lock(g_tuned_mutex);
+        g_tuned[TuneKey{P.ra.w, P.ra.h, P.ra.n_aux, P.window}] = pick;
+    }
+    if (times_ms) std::memcpy(times_ms, ms, sizeof(ms));
+    if (best) *best = pick;
     return ISTI_OK;
 }
 

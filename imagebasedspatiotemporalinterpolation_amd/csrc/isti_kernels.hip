@@ -4,9 +4,10 @@
 //   reproject_splat_kernel  K1, every source view of the frame in ONE launch
 //                           (renderer.cu:76-146, launched 2+2*n_aux times
 //                           by the reference, :643-649)
-//   resolve_kernel<W, AV>   K2: window init, fixed-point search, decision,
-//                           second chance, soft shadows, aux fallback, hole
-//                           retention (renderer.cu:148-465)
+//   resolve_*_kernel        K2 (renderer.cu:148-465) over a batch work table
+//                           and with learned starts; the per-render resolve
+//                           kernels are in resolve_kernels.hip, the shared
+//                           device code of K2 in resolve_device.hpp
 //   row_sum_kernel,
 //   mean_kernel             deterministic mean z (replaces the float
 //                           atomicAdd of image_buffer.cu:27-36,80-101)
@@ -17,25 +18,9 @@
 // deals workgroups round-robin over the 8 XCDs, each with its own L2), which
 // keeps the neighbourhood gathers of adjacent tiles in one L2.
 
-#include "isti_kernels.hpp"
+#include "resolve_device.hpp"
 
 namespace isti {
-
-// The explicit kernel arguments start at offset 0 of the kernarg segment.
-template <typename T>
-using KernArg = __attribute__((address_space(4))) const T;
-template <typename T>
-__device__ __forceinline__ KernArg<T> *kernarg_ptr() {
-    return (KernArg<T> *)__builtin_amdgcn_kernarg_segment_ptr();
-}
-
-// Block id -> logical tile id such that each XCD owns one contiguous range.
-__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
-    constexpr uint32_t NX = 8;
-    const uint32_t q = nb / NX, r = nb % NX;
-    const uint32_t x = b % NX, k = b / NX;
-    return x * q + (x < r ? x : r) + k;
-}
 
 // ---------------------------------------------------------------------------
 // Splat clear: 16-byte stores over every buffer of the frame.
@@ -116,510 +101,6 @@ __global__ __launch_bounds__(256) void reproject_splat_kernel(const SplatArgs a)
         if (b >= a.job[i].block_begin) j = i;
     const SplatJob &J = a.job[j];
     splat_pixel(J, a.vp_cam, a.vp_main[J.main_slot], a.init_linear != 0, (b - J.block_begin) * SPLAT_BLOCK + threadIdx.x);
-}
-
-// ---------------------------------------------------------------------------
-// K2: resolve (renderer.cu:148-465)
-// ---------------------------------------------------------------------------
-struct SplatInfo {
-    float depth;
-    uint32_t x, y;
-};
-
-// initializeSearchFromSplatting<W>, renderer.cu:148-175, over splat cells
-// staged in LDS (row length rw; (lx, ly) is the centre). Cells outside the
-// view were staged as the empty key, whose negative depth never wins, which
-// is the reference's `continue` for out-of-range cells.
-template <int W>
-__device__ __forceinline__ SplatInfo init_search_lds(const unsigned long long *t, int rw, int lx, int ly) {
-    const unsigned long long c = t[ly * rw + lx];
-    SplatInfo best{key_depth(c), key_x(c), key_y(c)};
-#pragma unroll
-    for (int dy = -W; dy <= W; dy++) {
-#pragma unroll
-        for (int dx = -W; dx <= W; dx++) {
-            const unsigned long long k = t[(ly + dy) * rw + lx + dx];
-            const float d = key_depth(k);
-            if (d > 0.0f && (!__builtin_isfinite(best.depth) || best.depth < 0.0f || d < best.depth))
-                best = SplatInfo{d, key_x(k), key_y(k)};
-        }
-    }
-    return best;
-}
-
-// Stages the splat cells [ox, ox+rw) x [oy, oy+rh) of view v into LDS rows
-// of constant stride SPAN (rw <= SPAN).
-template <int SPAN>
-__device__ __forceinline__ void stage_splat(unsigned long long *dst, const ResolveView &v, int ox, int oy, int rw,
-                                            int rh) {
-    const int n = SPAN * rh;
-    for (int i = (int)threadIdx.x; i < n; i += (int)(TILE_W * TILE_H)) {
-        const int yy = i / SPAN, xx = i - yy * SPAN;
-        const int gx = ox + xx, gy = oy + yy;
-        const bool in = gx >= 0 && gx < (int)v.w && gy >= 0 && gy < (int)v.h;
-        if (xx < rw) dst[i] = in ? v.splat[(uint32_t)gy * v.w + (uint32_t)gx] : SPLAT_EMPTY;
-    }
-}
-
-// iterativeSearch<N>, renderer.cu:177-202. The reference always runs N
-// steps of q <- step(q). step() depends on q alone (the target pixel is
-// fixed), so once the sequence revisits a pixel it is periodic from there
-// on, and q_N can be read off the cycle without running the remaining steps:
-//   - an out-of-range candidate keeps q (the reference's `continue`), and an
-//     unchanged candidate is a fixed point: period 1, stop;
-//   - q_{i+1} == q_{i+1-p} for p = 2..4: with c_0 = q_{i+1} .. c_{p-1} = q_i,
-//     q_N = c_{(N-i-1) mod p}, taken from the last four positions.
-// This is bit-identical to the N-step loop. (At low resolution ~40% of the
-// aux searches settle into 2-cycles, which would otherwise run all N steps.)
-// Returns the cam_ff sample at the final pixel in `m`.
-__device__ __forceinline__ uint32_t pack_px(uint32_t x, uint32_t y) { return (y << 16) | x; }
-
-// One fixed-point search in flight. All searches of a pixel (fwd/bwd
-// keyframe of the main and every aux camera) advance in lock-step, so their
-// dependent gathers overlap instead of running one chain after another.
-struct Chain {
-    uint32_t h0, h1, h2, h3;  // q_i, q_{i-1}, q_{i-2}, q_{i-3} (packed)
-    float4 m;                 // cam_ff at h0
-    bool active;
-};
-constexpr uint32_t PX_NONE = 0xffffffffu;  // x, y <= 65534: never a packed pixel
-
-// A chain that does not run (no splat candidate, or the skipped fwd search)
-// parks at pixel (0, 0), so the unconditional reloads in chain_step stay
-// inside the view; its result is never used.
-__device__ __forceinline__ void chain_start(Chain &c, bool run, uint32_t x, uint32_t y, const ResolveView &v) {
-    c.h0 = run ? pack_px(x, y) : 0u;
-    c.h1 = c.h2 = c.h3 = PX_NONE;
-    c.active = run;
-    c.m = run ? at(v.cam_ff, v.w, x, y) : make_float4(0.f, 0.f, 0.f, 0.f);
-}
-
-// Step i (0-based) of n_iter: q_{i+1} = step(q_i), with the exits described above.
-__device__ __forceinline__ void chain_step(Chain &c, int i, int n_iter, float2 pcs_full, const ResolveView &v) {
-    if (!c.active) return;
-    const int2 t = cs2pixel(pcs_full.x - c.m.x, pcs_full.y - c.m.y, v.w, v.h);
-    if (!in_range(t, v.w, v.h)) {
-        c.active = false;
-        return;
-    }
-    const uint32_t nt = pack_px((uint32_t)t.x, (uint32_t)t.y);
-    if (nt == c.h0) {
-        c.active = false;
-        return;
-    }
-    const int p = nt == c.h1 ? 2 : (nt == c.h2 ? 3 : (nt == c.h3 ? 4 : 0));
-    if (p != 0) {
-        const int idx = p - 1 - (n_iter - (i + 1)) % p;  // c_r = h_{p-1-r}
-        const uint32_t q = idx == 0 ? c.h0 : (idx == 1 ? c.h1 : (idx == 2 ? c.h2 : c.h3));
-        c.h0 = q;
-        c.m = at(v.cam_ff, v.w, q & 0xFFFFu, q >> 16);
-        c.active = false;
-        return;
-    }
-    c.h3 = c.h2;
-    c.h2 = c.h1;
-    c.h1 = c.h0;
-    c.h0 = nt;
-    c.m = at(v.cam_ff, v.w, (uint32_t)t.x, (uint32_t)t.y);
-}
-
-// computeReprojError, renderer.cu:204-215
-__device__ __forceinline__ float reproj_error(uint2 q, const ResolveView &v, float2 pcs_cam, float4 m) {
-    float ex = cs_coord(q.x, v.inv_w) + m.x - pcs_cam.x;
-    float ey = cs_coord(q.y, v.inv_h) + m.y - pcs_cam.y;
-    ex *= v.rs_x;
-    ey *= v.rs_y;
-    return sqrtf(ex * ex + ey * ey);
-}
-
-// colorFromShadow(idx, ...), renderer.cu:217-268
-__device__ uchar4 color_from_shadow_1(uint32_t vidx, int sfpa, const ResolveView &v, uint32_t px, uint32_t py) {
-    const uchar4 main_color = fetch(v.img, v.w, v.h, px, py);
-    if (vidx < (uint32_t)sfpa && vidx > 0) {
-        const uchar4 si = fetch(v.sh_info, v.w, v.h, px, py);
-        const uint32_t bits = ((uint32_t)si.x << 16) | ((uint32_t)si.y << 8) | (uint32_t)si.z;
-        if (v.sh_b != nullptr && bits != 0) {
-            const int init_val = (int)(bits & SOFTSHADOW_VAL_MASK);
-            const int my_val = (int)(bits >> (SOFTSHADOW_BITS * vidx) & SOFTSHADOW_VAL_MASK);
-            int min_val = (int)SOFTSHADOW_VAL_MASK, max_val = 0;
-            for (int i = 0; i < sfpa; i++) {
-                const int c = (int)(bits >> (SOFTSHADOW_BITS * (uint32_t)i) & SOFTSHADOW_VAL_MASK);
-                min_val = c < min_val ? c : min_val;
-                max_val = c > max_val ? c : max_val;
-            }
-            const float t = (float)(my_val - min_val) / (float)(max_val - min_val);
-            const uchar4 cmin = min_val == init_val ? main_color : fetch(v.sh_a, v.w, v.h, px, py);
-            const uchar4 cmax = max_val == init_val ? main_color : fetch(v.sh_b, v.w, v.h, px, py);
-            const float omt = (float)(1.0 - (double)t);
-            const float rx = (float)cmin.x * omt + (float)cmax.x * t;
-            const float ry = (float)cmin.y * omt + (float)cmax.y * t;
-            const float rz = (float)cmin.z * omt + (float)cmax.z * t;
-            return make_uchar4(cvt_u8(rx), cvt_u8(ry), cvt_u8(rz), 255);
-        }
-    }
-    return main_color;
-}
-
-// colorFromShadow(idcs, t, ...), renderer.cu:270-295 (frameIdx2Idx :227-230)
-__device__ uchar4 color_from_shadow(uint2 idcs, float st, int sfpa, bool no_shadows, const ResolveView &v,
-                                    uint32_t px, uint32_t py) {
-    if (no_shadows || idcs.x >= (uint32_t)sfpa || idcs.y == 0 || v.sh_a == nullptr)
-        return fetch(v.img, v.w, v.h, px, py);
-    const uint32_t vx = (uint32_t)(v.forward ? (int)idcs.x : sfpa - (int)idcs.x);
-    const uint32_t vy = (uint32_t)(v.forward ? (int)idcs.y : sfpa - (int)idcs.y);
-    if (vx == vy || vy >= (uint32_t)sfpa) return color_from_shadow_1(vx, sfpa, v, px, py);
-    if (vx >= (uint32_t)sfpa) return color_from_shadow_1(vy, sfpa, v, px, py);
-    const uchar4 ca = color_from_shadow_1(vx, sfpa, v, px, py);
-    const uchar4 cb = color_from_shadow_1(vy, sfpa, v, px, py);
-    return lerp_to_u8(ca, cb, st);
-}
-
-struct FrameConst {
-    float alpha;
-    int fpa, sfpa;
-    bool no_shadows;
-    uint32_t frame_idx;
-    uint2 sidx;
-    float st;
-};
-
-// secondChanceAndBlend_kernel<EPS_2>, renderer.cu:297-327
-__device__ uchar4 second_chance(float weight, const FrameConst &fc, SplatInfo si, const ResolveView &from,
-                                const ResolveView &to, float eps2) {
-    const float4 m_ft = fetch(from.full_ff, from.w, from.h, si.x, si.y);
-    const float sz = fetch(from.z, from.w, from.h, si.x, si.y);
-    const float proj_x = cs_coord(si.x, from.inv_w) + m_ft.x, proj_y = cs_coord(si.y, from.inv_h) + m_ft.y;
-    const float proj_z = sz + m_ft.z;
-    const int2 pp = cs2pixel(proj_x, proj_y, to.w, to.h);
-
-    uchar4 out = color_from_shadow(fc.sidx, fc.st, fc.sfpa, fc.no_shadows, from, si.x, si.y);
-    if (fc.frame_idx >= (uint32_t)fc.fpa) return out;
-    if (in_range(pp, to.w, to.h)) {
-        const uint32_t ux = (uint32_t)pp.x, uy = (uint32_t)pp.y;
-        const float zt = fetch(to.z, to.w, to.h, ux, uy);
-        const float4 m_tf = fetch(to.full_ff, to.w, to.h, ux, uy);
-        const float dx = m_ft.x - (-m_tf.x), dy = m_ft.y - (-m_tf.y), dz = m_ft.z - (-m_tf.z);
-        const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
-        if (dist < EPS_3 && fabsf(zt - proj_z) < eps2)
-            out = lerp_to_u8(color_from_shadow(fc.sidx, fc.st, fc.sfpa, fc.no_shadows, to, ux, uy), out, weight);
-    }
-    return out;
-}
-
-// ReprojResult (renderer.cu:49-57) without its colour: the colour depends
-// only on the chosen candidate, so it is computed once, for the candidate
-// that is finally written (the reference computes it for the main camera and
-// for every aux camera).
-struct Cand {
-    bool valid;
-    bool visible_in_main;
-    int priority;
-    float error;
-    float depth;
-    int dir;  // 0: fwd keyframe, 1: bwd keyframe, 2: 0.5 blend of both (ALWAYS_VALID)
-};
-
-// Forward/backward decision of one camera, renderer.cu:366-396.
-template <bool ALWAYS_VALID>
-__device__ __forceinline__ Cand decide(float fe, float be, float fd, float bd, bool fv, bool bv, int fprio, int bprio,
-                                       float eps1, float eps2) {
-    Cand k{false, false, 0, 0.f, 0.f, 0};
-    if (fe < eps1 || be < eps1) {
-        const bool both = fe < eps1 && be < eps1;
-        const bool similar = fabsf(depth_to_z(fd) - depth_to_z(bd)) < eps2;
-        const bool choose_fwd = (both && !similar && fd < bd) || ((!both || similar) && fe < be);
-        k.valid = true;
-        k.visible_in_main = choose_fwd ? (fv || (both && similar && bv)) : (bv || (both && similar && fv));
-        k.priority = choose_fwd ? fprio : bprio;
-        k.error = choose_fwd ? fe : be;
-        k.depth = choose_fwd ? fd : bd;
-        k.dir = choose_fwd ? 0 : 1;
-    } else if (ALWAYS_VALID) {
-        k.valid = true;
-        k.dir = 2;
-    }
-    return k;
-}
-
-// Colour of a candidate of camera (fwd, bwd): renderer.cu:378-396.
-__device__ __forceinline__ uchar4 cand_colour(const Cand &k, const FrameConst &fc, SplatInfo fi, SplatInfo bi,
-                                              const ResolveView &fwd, const ResolveView &bwd, float eps2) {
-    if (k.dir == 0) return second_chance(1.0f - fc.alpha, fc, fi, fwd, bwd, eps2);
-    if (k.dir == 1) return second_chance(fc.alpha, fc, bi, bwd, fwd, eps2);
-    const uchar4 cf = second_chance(1.0f - fc.alpha, fc, fi, fwd, bwd, eps2);
-    const uchar4 cb = second_chance(fc.alpha, fc, bi, bwd, fwd, eps2);
-    return lerp_to_u8(cf, cb, 0.5f);
-}
-
-// bidirectionalReprojection_kernel<SW, SI, MAV>, renderer.cu:400-465, with
-// bidirectionalReprojection (:329-398) evaluated for the main camera and
-// NAUX aux cameras at once:
-//   1. splat-window initialisation of every (camera, keyframe) search;
-//   2. all searches in lock-step (see Chain);
-//   3. per-camera forward/backward decision (:366-396);
-//   4. aux selection and main/aux choice (:429-464);
-//   5. second chance + blend + soft shadows for the written candidate only.
-// Pixels without a valid result are not written (hole retention).
-template <int W, int NAUX>
-struct ResolveLds {
-    static constexpr int NC = 1 + NAUX;
-    static constexpr int SPAN = TILE_W + 2 * W;  // staged cells per axis (TILE_W == TILE_H)
-    unsigned long long splat[NC][2][SPAN * SPAN];
-};
-
-// One 16x16 tile of one render. Every thread of the block must call it (the
-// LDS staging ends in a barrier); pixels outside the image return early.
-template <int W, bool MAV, int NAUX, bool LEARNED = false, typename A>
-__device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveLds<W, NAUX> &lds) {
-    constexpr int NC = 1 + NAUX;
-    constexpr int SPAN = TILE_W + 2 * W;
-    auto &s_splat = lds.splat;
-    const uint32_t ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
-    const uint32_t x0 = tx * TILE_W, y0 = ty * TILE_H;
-    const uint32_t x = x0 + (threadIdx.x & (TILE_W - 1));
-    const uint32_t y = y0 + (threadIdx.x / TILE_W);
-
-    // Stage every splat cell the tile's windows read: camera c covers camera
-    // pixels [x0/rx, xl/rx] (xl = last in-image column of the tile) plus a
-    // W-cell border, at most SPAN x SPAN cells.
-    int org_x[NC], org_y[NC], span_x[NC];
-#pragma unroll
-    for (int c = 0; c < NC; c++) {
-        const uint32_t rx = a.w / a.fwd[c].w, ry = a.h / a.fwd[c].h;
-        const uint32_t xl = min(x0 + TILE_W - 1, a.w - 1), yl = min(y0 + TILE_H - 1, a.h - 1);
-        org_x[c] = (int)(x0 / rx) - W;
-        org_y[c] = (int)(y0 / ry) - W;
-        span_x[c] = (int)(xl / rx - x0 / rx) + 1 + 2 * W;
-        const int span_y = (int)(yl / ry - y0 / ry) + 1 + 2 * W;
-        stage_splat<SPAN>(s_splat[c][0], a.fwd[c], org_x[c], org_y[c], span_x[c], span_y);
-        stage_splat<SPAN>(s_splat[c][1], a.bwd[c], org_x[c], org_y[c], span_x[c], span_y);
-    }
-    __syncthreads();
-    if (x >= a.w || y >= a.h) return;
-
-    FrameConst fc;
-    fc.alpha = a.alpha;
-    fc.fpa = a.fpa;
-    fc.sfpa = a.sfpa;
-    fc.no_shadows = a.no_shadows != 0;
-    fc.frame_idx = f2u_rn((float)a.fpa * a.alpha);
-    fc.sidx = make_uint2(f2u_rd((float)a.sfpa * a.alpha + 1e-9f), f2u_ru((float)a.sfpa * a.alpha - 1e-9f));
-    fc.st = (float)a.sfpa * a.alpha - (float)fc.sidx.x;
-
-    const float2 pcs = make_float2(cs_coord(x, a.fwd[0].inv_w), cs_coord(y, a.fwd[0].inv_h));
-    // per camera: pixel in the camera's own resolution (renderer.cu:432-434)
-    uint32_t cx[NC], cy[NC];
-    float2 cpcs[NC];
-    cx[0] = x;
-    cy[0] = y;
-    cpcs[0] = pcs;
-#pragma unroll
-    for (int c = 1; c < NC; c++) {
-        cx[c] = x / (a.w / a.fwd[c].w);
-        cy[c] = y / (a.h / a.fwd[c].h);
-        cpcs[c] = make_float2(cs_coord(cx[c], a.fwd[c].inv_w), cs_coord(cy[c], a.fwd[c].inv_h));
-    }
-
-    // 1. window initialisation from LDS
-    SplatInfo si[NC][2];
-#pragma unroll
-    for (int c = 0; c < NC; c++) {
-        const int lx = (int)cx[c] - org_x[c], ly = (int)cy[c] - org_y[c];
-        si[c][0] = init_search_lds<W>(s_splat[c][0], SPAN, lx, ly);
-        si[c][1] = init_search_lds<W>(s_splat[c][1], SPAN, lx, ly);
-    }
-    // 2. searches in lock-step; the fwd search is skipped for alpha >= 1 (:335)
-    Chain ch[NC][2];
-    bool ran[NC][2];
-    const SplatInfo window0[2] = {si[0][0], si[0][1]};  // main camera's window starts
-    bool learned[2] = {false, false};
-#pragma unroll
-    for (int c = 0; c < NC; c++) {
-        ran[c][0] = si[c][0].depth > 0.0f && fc.alpha < 1.0f;
-        ran[c][1] = si[c][1].depth > 0.0f;
-        if (LEARNED && c == 0) {
-            // learned start: the network's offset from this pixel, where it is confident
-#pragma unroll
-            for (int d = 0; d < 2; d++) {
-                if (d == 0 && !(fc.alpha < 1.0f)) continue;
-                const ResolveView &v = d ? a.bwd[0] : a.fwd[0];
-                const float *o = a.net_out + (((size_t)d * a.net_h + (y >> a.net_shift)) * a.net_w + (x >> a.net_shift)) * 16;
-                if (o[2] >= a.conf_logit) {
-                    // one analytic search step from the pixel itself (p - cam_ff(p)),
-                    // corrected by the network's residual in cells of 2^net_shift pixels
-                    const float cell = (float)(1u << a.net_shift);
-                    const float4 m = at(v.cam_ff, v.w, x, y);
-                    float fx = ((float)x + 0.5f - m.x * (0.5f * (float)v.w)) + o[0] * cell;
-                    float fy = ((float)y + 0.5f - m.y * (0.5f * (float)v.h)) + o[1] * cell;
-                    fx = fx >= 0.0f ? (fx <= (float)(v.w - 1) ? fx : (float)(v.w - 1)) : 0.0f;  // NaN -> 0
-                    fy = fy >= 0.0f ? (fy <= (float)(v.h - 1) ? fy : (float)(v.h - 1)) : 0.0f;
-                    si[0][d].x = (uint32_t)fx;
-                    si[0][d].y = (uint32_t)fy;
-                    ran[0][d] = true;
-                    learned[d] = true;
-                }
-            }
-        }
-        chain_start(ch[c][0], ran[c][0], si[c][0].x, si[c][0].y, a.fwd[c]);
-        chain_start(ch[c][1], ran[c][1], si[c][1].x, si[c][1].y, a.bwd[c]);
-    }
-    uint32_t steps = 0;  // main-camera search steps taken (learned statistics)
-    for (int i = 0; i < a.max_iter; i++) {
-        bool any = false;
-#pragma unroll
-        for (int c = 0; c < NC; c++) any = any || ch[c][0].active || ch[c][1].active;
-        if (!any) break;
-        if (LEARNED) steps += (uint32_t)ch[0][0].active + (uint32_t)ch[0][1].active;
-#pragma unroll
-        for (int c = 0; c < NC; c++) {
-            chain_step(ch[c][0], i, a.max_iter, pcs, a.fwd[c]);
-            chain_step(ch[c][1], i, a.max_iter, pcs, a.bwd[c]);
-        }
-    }
-    // depth, reprojection error and visibility at the found pixels (:338-341)
-    float err[NC][2];
-    bool vis[NC][2];
-#pragma unroll
-    for (int c = 0; c < NC; c++) {
-#pragma unroll
-        for (int d = 0; d < 2; d++) {
-            const ResolveView &v = d ? a.bwd[c] : a.fwd[c];
-            err[c][d] = FLT_MAX;
-            vis[c][d] = false;
-            if (ran[c][d]) {
-                const uint32_t qx = ch[c][d].h0 & 0xFFFFu, qy = ch[c][d].h0 >> 16;
-                const float4 m = ch[c][d].m;
-                si[c][d].x = qx;
-                si[c][d].y = qy;
-                si[c][d].depth = z_to_depth(at(v.z, v.w, qx, qy) + m.z);
-                err[c][d] = reproj_error(make_uint2(qx, qy), v, cpcs[c], m);
-                vis[c][d] = m.w > 0.0f;
-            }
-        }
-    }
-    if (LEARNED) {
-        // the plain search judges the network: a learned start is redone from
-        // the window start unless its search ends below EPS_1 on a surface
-        // not behind the window's nearest splat candidate (the splat's depth
-        // test; a fixed point on an occluded surface is valid but hidden)
-        bool redo[2];
-#pragma unroll
-        for (int d = 0; d < 2; d++) {
-            const bool hidden = window0[d].depth > 0.0f &&
-                                !(depth_to_z(si[0][d].depth) - depth_to_z(window0[d].depth) < EPS_2_FULL_RES);
-            redo[d] = learned[d] && !(err[0][d] < EPS_1_FULL_RES && !hidden);
-            if (redo[d]) {
-                si[0][d] = window0[d];
-                ran[0][d] = window0[d].depth > 0.0f && (d == 1 || fc.alpha < 1.0f);
-            }
-            chain_start(ch[0][d], redo[d] && ran[0][d], si[0][d].x, si[0][d].y, d ? a.bwd[0] : a.fwd[0]);
-        }
-        for (int i = 0; i < a.max_iter; i++) {
-            if (!(ch[0][0].active || ch[0][1].active)) break;
-            steps += (uint32_t)ch[0][0].active + (uint32_t)ch[0][1].active;
-            chain_step(ch[0][0], i, a.max_iter, pcs, a.fwd[0]);
-            chain_step(ch[0][1], i, a.max_iter, pcs, a.bwd[0]);
-        }
-#pragma unroll
-        for (int d = 0; d < 2; d++) {
-            if (!redo[d]) continue;
-            const ResolveView &v = d ? a.bwd[0] : a.fwd[0];
-            err[0][d] = FLT_MAX;
-            vis[0][d] = false;
-            if (ran[0][d]) {
-                const uint32_t qx = ch[0][d].h0 & 0xFFFFu, qy = ch[0][d].h0 >> 16;
-                const float4 m = ch[0][d].m;
-                si[0][d].x = qx;
-                si[0][d].y = qy;
-                si[0][d].depth = z_to_depth(at(v.z, v.w, qx, qy) + m.z);
-                err[0][d] = reproj_error(make_uint2(qx, qy), v, cpcs[0], m);
-                vis[0][d] = m.w > 0.0f;
-            }
-        }
-        if (a.learned_stats) {  // one atomic per wave and counter (ballots see active lanes only)
-            const bool fwd_ok = fc.alpha < 1.0f;
-            const uint32_t kept = (uint32_t)(__popcll(__ballot(learned[0] && !redo[0])) +
-                                             __popcll(__ballot(learned[1] && !redo[1])));
-            const uint32_t rej = (uint32_t)(__popcll(__ballot(redo[0])) + __popcll(__ballot(redo[1])));
-            const uint32_t low = (uint32_t)(__popcll(__ballot(fwd_ok && !learned[0])) + __popcll(__ballot(!learned[1])));
-            const uint32_t tot = (uint32_t)(__popcll(__ballot(fwd_ok)) + __popcll(__ballot(1)));
-            uint32_t wave_steps = 0;  // sum over the active lanes, one ballot per bit
-#pragma unroll
-            for (int b = 0; b < 16; b++) wave_steps += (uint32_t)__popcll(__ballot((steps >> b) & 1u)) << b;
-            const unsigned long long act = __ballot(1);
-            if ((int)(threadIdx.x & 63) == __ffsll((long long)act) - 1) {
-                atomicAdd(&a.learned_stats[0], kept);
-                atomicAdd(&a.learned_stats[1], rej);
-                atomicAdd(&a.learned_stats[2], low);
-                atomicAdd(&a.learned_stats[3], tot);
-                atomicAdd(&a.learned_stats[4], wave_steps);
-            }
-        }
-    }
-    if (a.q_out) {  // capture of the main camera's search results
-#pragma unroll
-        for (int d = 0; d < 2; d++)
-            a.q_out[2 * (y * a.w + x) + d] = err[0][d] < EPS_1_FULL_RES ? pack_px(si[0][d].x, si[0][d].y) : 0xffffffffu;
-    }
-    // 3.-4. per-camera decision (:366-396), folded at once into the running
-    // aux selection (:429-451) so no per-camera array outlives the loop
-    // (a dynamically indexed array would live in scratch memory).
-    Cand main_k = decide<MAV>(err[0][0], err[0][1], si[0][0].depth, si[0][1].depth, vis[0][0], vis[0][1],
-                              a.fwd[0].priority, a.bwd[0].priority, EPS_1_FULL_RES, EPS_2_FULL_RES);
-    bool aux_valid = false;
-    Cand aux_k = Cand{false, false, 0, 0.f, 0.f, 0};
-    SplatInfo aux_f = si[0][0], aux_b = si[0][1];
-    int aux_c = 0;
-#pragma unroll
-    for (int c = 1; c < NC; c++) {
-        const Cand kc = decide<false>(err[c][0], err[c][1], si[c][0].depth, si[c][1].depth, vis[c][0], vis[c][1],
-                                      a.fwd[c].priority, a.bwd[c].priority, EPS_1_LOW_RES, EPS_2_LOW_RES);
-        bool take = false;
-        if (!aux_valid) {
-            take = kc.valid;  // an invalid first result leaves "no valid best"
-        } else if (kc.valid) {
-            const bool similar = fabsf(depth_to_z(aux_k.depth) - depth_to_z(kc.depth)) < EPS_2_LOW_RES;
-            take = (!similar && kc.depth < aux_k.depth) ||
-                   (similar && kc.priority >= aux_k.priority && kc.error < aux_k.error);
-        }
-        if (take) {
-            aux_valid = true;
-            aux_k = kc;
-            aux_f = si[c][0];
-            aux_b = si[c][1];
-            aux_c = c;
-        }
-    }
-    // main vs aux (:453-464)
-    bool use_aux = false;
-    if (aux_valid) {
-        use_aux = !main_k.valid;
-        if (main_k.valid && !aux_k.visible_in_main) {
-            const bool similar = fabsf(depth_to_z(aux_k.depth) - depth_to_z(main_k.depth)) < EPS_2_LOW_RES;
-            use_aux = !similar && aux_k.depth < main_k.depth;
-        }
-    }
-    if (!use_aux && !main_k.valid) return;  // hole: keep the previous frame's pixel
-    // 5. colour of the written candidate
-    const int wc = use_aux ? aux_c : 0;
-    const Cand wk = use_aux ? aux_k : main_k;
-    const SplatInfo wf = use_aux ? aux_f : si[0][0];
-    const SplatInfo wb = use_aux ? aux_b : si[0][1];
-    const uchar4 col = cand_colour(wk, fc, wf, wb, a.fwd[wc], a.bwd[wc], wc == 0 ? EPS_2_FULL_RES : EPS_2_LOW_RES);
-    a.out[y * a.out_pitch_px + x] = col;
-}
-
-template <int W, bool MAV, int NAUX>
-__global__ __launch_bounds__(256) void resolve_kernel(const ResolveArgs args) {
-    // Read the argument block in place (constant address space) instead of
-    // through a by-value copy, which the compiler may spill to scratch.
-#if defined(__HIP_DEVICE_COMPILE__)
-    const KernArg<ResolveArgs> &a = *kernarg_ptr<ResolveArgs>();
-#else
-    const ResolveArgs &a = args;  // host pass only parses the body
-#endif
-    __shared__ ResolveLds<W, NAUX> lds;
-    resolve_tile<W, MAV, NAUX>(a, xcd_remap(blockIdx.x, a.n_tiles), lds);
 }
 
 // Learned search starts: the same resolve, with the main camera's searches
@@ -909,29 +390,6 @@ hipError_t launch_splat(const SplatArgs &a, hipStream_t s) {
     if (a.n_blocks == 0) return hipSuccess;
     hipLaunchKernelGGL(reproject_splat_kernel, dim3(a.n_blocks), dim3(SPLAT_BLOCK), 0, s, a);
     return hipGetLastError();
-}
-
-template <int W, bool MAV>
-static hipError_t launch_resolve_n(const ResolveArgs &a, hipStream_t s) {
-    const dim3 grid(a.n_tiles), block(TILE_W * TILE_H);
-    switch (a.n_aux) {
-        case 0: hipLaunchKernelGGL((resolve_kernel<W, MAV, 0>), grid, block, 0, s, a); break;
-        case 1: hipLaunchKernelGGL((resolve_kernel<W, MAV, 1>), grid, block, 0, s, a); break;
-        case 2: hipLaunchKernelGGL((resolve_kernel<W, MAV, 2>), grid, block, 0, s, a); break;
-        case 3: hipLaunchKernelGGL((resolve_kernel<W, MAV, 3>), grid, block, 0, s, a); break;
-        default: hipLaunchKernelGGL((resolve_kernel<W, MAV, 4>), grid, block, 0, s, a); break;
-    }
-    return hipGetLastError();
-}
-
-hipError_t launch_resolve(const ResolveArgs &a, int window, bool always_valid, hipStream_t s) {
-    if (a.n_tiles == 0) return hipSuccess;
-    if (a.n_aux < 0 || a.n_aux > ISTI_MAX_AUX) return hipErrorInvalidValue;
-    // the reference pairs window 1 with plain validity and window 3 with
-    // main-always-valid (renderer.cu:678-689); those are the two variants built
-    if (window == 1 && !always_valid) return launch_resolve_n<1, false>(a, s);
-    if (window == 3 && always_valid) return launch_resolve_n<3, true>(a, s);
-    return hipErrorInvalidValue;
 }
 
 hipError_t launch_features(const FeatureArgs &a, hipStream_t s) {

@@ -16,7 +16,13 @@ namespace isti {
 constexpr int MAX_VIEWS = 1 + ISTI_MAX_AUX;        // main + aux per slot
 constexpr int MAX_SPLAT_JOBS = 2 * MAX_VIEWS;       // (slot, view) pairs
 constexpr uint32_t SPLAT_BLOCK = 256;               // source pixels per K1 block
-constexpr uint32_t TILE_W = 16, TILE_H = 16;        // resolve tile (one block)
+constexpr uint32_t TILE_W = 16, TILE_H = 16;        // default resolve tile (one block)
+constexpr uint32_t RESOLVE_THREADS = TILE_W * TILE_H;
+// Resolve tile shapes the single-render path is built for (index = shape id;
+// every shape has RESOLVE_THREADS pixels and gives identical results).
+constexpr int N_TILE_SHAPES = 4;
+constexpr uint32_t TILE_SHAPE_W[N_TILE_SHAPES] = {16, 32, 64, 8};
+constexpr uint32_t TILE_SHAPE_H[N_TILE_SHAPES] = {16, 8, 4, 32};
 
 // One source view of the reproject+splat stage (K1, renderer.cu:76-146).
 struct SplatJob {
@@ -151,7 +157,7 @@ hipError_t launch_resolve_learned(const ResolveArgs &a, hipStream_t s);
 
 hipError_t launch_clear(const ClearArgs &a, hipStream_t s);
 hipError_t launch_splat(const SplatArgs &a, hipStream_t s);
-hipError_t launch_resolve(const ResolveArgs &a, int window, bool always_valid, hipStream_t s);
+hipError_t launch_resolve(const ResolveArgs &a, int window, bool always_valid, int shape, hipStream_t s);
 hipError_t launch_mean_z(const float *z, uint32_t w, uint32_t h, double *row_sums, float *mean, hipStream_t s);
 hipError_t launch_timewarp(const TimewarpArgs &a, hipStream_t s);
 

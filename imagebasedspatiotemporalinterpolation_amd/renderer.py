@@ -191,6 +191,25 @@ class Renderer:
         N.check(N.lib().isti_render(self._h, ctypes.byref(ci), render_buffer.data_ptr(), w * 4,
                                     ctypes.c_void_p(s.cuda_stream)), self._h)
 
+    def autotune(self, render_buffer, render_info: RenderInfo, reps: int = 20, stream=None) -> tuple:
+        """Times every resolve tile shape on this frame and keeps the fastest
+        for renders of its size, aux count and window; returns (best shape,
+        per-shape ms). Synchronises the stream."""
+        import torch
+
+        h, w = render_buffer.shape[:2]
+        s = stream if stream is not None else torch.cuda.current_stream()
+        ci = render_info.to_c()
+        ms = (ctypes.c_float * N.ISTI_N_TILE_SHAPES)()
+        best = ctypes.c_int32(-1)
+        N.check(N.lib().isti_autotune(self._h, ctypes.byref(ci), render_buffer.data_ptr(), w * 4, int(reps), ms,
+                                      ctypes.byref(best), ctypes.c_void_p(s.cuda_stream)), self._h)
+        return best.value, [float(v) for v in ms]
+
+    def set_tile_shape(self, shape: int) -> None:
+        """Force a resolve tile shape (index into _native.TILE_SHAPES), or -1 for the tuned one."""
+        N.check(N.lib().isti_set_tile_shape(self._h, int(shape)), self._h)
+
     def swap(self) -> None:
         """FrameData::swap on the renderer's own keyframe descriptors."""
         N.check(N.lib().isti_keyframe_swap(self._h), self._h)

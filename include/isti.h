@@ -164,6 +164,25 @@ isti_status isti_keyframe_swap(isti_renderer *r);
 isti_status isti_render(isti_renderer *r, const isti_render_info *info,
                         uint8_t *out_rgba8, size_t out_pitch_bytes, void *stream);
 
+/* Resolve tile shapes (render pixels per workgroup, 256 each): 0 = 16x16,
+ * 1 = 32x8, 2 = 64x4, 3 = 8x32. Every shape gives identical results; they
+ * differ in speed (halo staging, gather locality, wave shape). No counterpart
+ * in the reference (16x16 blocks, src/renderer.cu:676-693). */
+#define ISTI_N_TILE_SHAPES 4
+
+/* Autotuner: renders frame `info` once, then times the resolve pass of every
+ * tile shape (`reps` launches each, HIP events) and keeps the fastest for
+ * every later isti_render of the same render size, aux count and search
+ * window, in this process. times_ms (optional, ISTI_N_TILE_SHAPES floats)
+ * receives the average resolve time per shape, *best (optional) the shape
+ * chosen. `out_rgba8` receives the frame. Synchronises `stream`. */
+isti_status isti_autotune(isti_renderer *r, const isti_render_info *info, uint8_t *out_rgba8,
+                          size_t out_pitch_bytes, int32_t reps, float *times_ms, int32_t *best, void *stream);
+
+/* Forces the resolve tile shape of r's renders (0..ISTI_N_TILE_SHAPES-1);
+ * -1 returns to the tuned shape, or 16x16 where nothing was tuned. */
+isti_status isti_set_tile_shape(isti_renderer *r, int32_t shape);
+
 /* Device scratch of (slot, view) from the last render (see above). */
 isti_status isti_scratch(const isti_renderer *r, int32_t slot, int32_t view,
                          isti_scratch_view *out);
