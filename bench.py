@@ -18,6 +18,7 @@ algorithm) on a bounded sample of the same frames.
 from __future__ import annotations
 
 import argparse
+import gc
 import ctypes
 import json
 import os
@@ -196,6 +197,7 @@ def learned_bench(n_frames: int, seed: int, iters: int) -> dict:
             fn(infos[i % len(infos)])
         torch.cuda.synchronize()
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n_frames)]
+        gc.disable()  # no collector pauses between the enqueued frames
         t0 = time.perf_counter()
         for i in range(n_frames):
             ev[i][0].record(stream)
@@ -203,6 +205,7 @@ def learned_bench(n_frames: int, seed: int, iters: int) -> dict:
             ev[i][1].record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        gc.enable()
         ms = np.array([a.elapsed_time(b) for a, b in ev])
         return {"mpix_per_s": round(n_frames * W * H / wall / 1e6, 1),
                 "p50_frame_ms": round(float(np.percentile(ms, 50)), 4),
@@ -307,6 +310,7 @@ def main():
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n_frames)]
     rend.set_stage_timing(True)
     torch.cuda.synchronize()
+    gc.disable()
     t0 = time.perf_counter()
     k = 0
     for _ in range(args.steps):
@@ -317,6 +321,7 @@ def main():
             k += 1
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    gc.enable()
     totals, count = rend.stage_totals()
     rend.set_stage_timing(False)
     frame_ms = np.array([a.elapsed_time(b) for a, b in ev])

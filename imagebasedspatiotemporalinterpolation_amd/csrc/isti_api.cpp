@@ -141,7 +141,20 @@ ResolveView resolve_view(const isti_view_desc &d, const Scratch &s, bool forward
     const float mn = (float)(d.width < d.height ? d.width : d.height);
     v.rs_x = (float)d.width / mn;
     v.rs_y = (float)d.height / mn;
+    v.ratio_x = v.ratio_y = 1;
+    v.magic_x = v.magic_y = 0;
     return v;
+}
+
+// Multiply-high divisor of div_small: floor(2^32 / r) + 1 (unused for r = 1).
+uint32_t div_magic(uint32_t r) { return r <= 1 ? 0u : (uint32_t)((1ull << 32) / r + 1ull); }
+
+// Render-to-view pixel ratio of a resolve view (renderer.cu:433).
+void set_ratio(ResolveView &v, uint32_t render_w, uint32_t render_h) {
+    v.ratio_x = render_w / v.w;
+    v.ratio_y = render_h / v.h;
+    v.magic_x = div_magic(v.ratio_x);
+    v.magic_y = div_magic(v.ratio_y);
 }
 
 isti_status validate(isti_renderer *r, const isti_render_info *info, size_t out_pitch) {
@@ -316,6 +329,8 @@ isti_status plan_frame(isti_renderer *r, const isti_render_info *info, uint8_t *
         const int prio_b = v == 0 ? 0 : r->views[1][v].priority;
         ra.fwd[v] = resolve_view(r->views[0][v], r->scratch[0][v], true, prio_f);
         ra.bwd[v] = resolve_view(r->views[1][v], r->scratch[1][v], false, prio_b);
+        set_ratio(ra.fwd[v], info->width, info->height);
+        set_ratio(ra.bwd[v], info->width, info->height);
     }
     ra.out = reinterpret_cast<uchar4 *>(out);
     ra.out_pitch_px = (uint32_t)(out_pitch / 4);

@@ -104,12 +104,27 @@ __device__ __forceinline__ float4 project(const Mat4 &M, float4 v) {
 __device__ __forceinline__ float z_to_depth(float z) { return (z + 1.0f) * 0.5f; }
 __device__ __forceinline__ float depth_to_z(float d) { return d * 2.0f - 1.0f; }
 
-// helper_common.h:22-38: lerp_color then color_to_uchar4 (to_8bit truncates)
+// helper_common.h:22-38: lerp_color then color_to_uchar4 (to_8bit truncates).
+// The division v / 255 is evaluated as RN32(RN64(v) * RN64(1/255)): with
+// v = M 2^e (M < 2^24) and q = v / 255 in [2^E, 2^(E+1)), q - m for any
+// binary32 rounding midpoint m is a non-zero multiple of 2^(E-24) / 255
+// (v and 255 m are multiples of 2^(E-24); v / 255 = m would need 255 | M and
+// then q is representable), so |q - m| >= 2^(E-32), while the double product
+// is within 2^(E-51) of q: both round to the same binary32 (subnormal q
+// likewise, on the fixed 2^-149 grid).
+constexpr double INV_255 = 1.0 / 255.0;
 __device__ __forceinline__ uint8_t lerp_ch(uint32_t ua, uint32_t ub, float t) {
     const float fa = (float)ua, fb = (float)ub;
-    const float l = (fa + t * (fb - fa)) / 255.f;
+    const float l = (float)((double)(fa + t * (fb - fa)) * INV_255);
     const int32_t v = cvt_s32(l * 255.f);
     return (uint8_t)(v < 0 ? 0 : (v > 255 ? 255 : v));
+}
+
+// floor(x / r) for x, r < 2^16 as a multiply-high by m = floor(2^32 / r) + 1
+// (m unused for r = 1): with m r = 2^32 + e, 0 < e <= r, the error term
+// x e / 2^32 < 1 / r never reaches the next integer.
+__device__ __forceinline__ uint32_t div_small(uint32_t x, uint32_t r, uint32_t m) {
+    return r == 1u ? x : __umulhi(x, m);
 }
 __device__ __forceinline__ uchar4 lerp_to_u8(uchar4 a, uchar4 b, float t) {
     return make_uchar4(lerp_ch(a.x, b.x, t), lerp_ch(a.y, b.y, t), lerp_ch(a.z, b.z, t), lerp_ch(a.w, b.w, t));
@@ -128,6 +143,11 @@ __device__ __forceinline__ uint32_t key_y(unsigned long long k) { return (uint32
 template <typename T>
 __device__ __forceinline__ T at(const T *plane, uint32_t w, uint32_t x, uint32_t y) {
     return plane[y * w + x];
+}
+
+// The xy half of a float4 plane's sample (one 8-byte load).
+__device__ __forceinline__ float2 at_xy(const float4 *plane, uint32_t w, uint32_t x, uint32_t y) {
+    return reinterpret_cast<const float2 *>(plane)[2u * (y * w + x)];
 }
 
 // Point sample with border addressing (src/util/image_buffer.cu:227-249):

@@ -67,6 +67,9 @@ struct ResolveView {
     int32_t forward;
     double inv_w, inv_h;  // RN64(1/w), RN64(1/h) for pixel2CS
     float rs_x, rs_y;     // w / min(w, h), h / min(w, h) (renderer.cu:213)
+    // render size / view size per axis (renderer.cu:433) and their
+    // multiply-high divisors (div_small)
+    uint32_t ratio_x, ratio_y, magic_x, magic_y;
 };
 
 struct ResolveArgs {

@@ -77,7 +77,8 @@ __device__ __forceinline__ void stage_splat(unsigned long long *dst, const Resol
 //     q_N = c_{(N-i-1) mod p}, taken from the last four positions.
 // This is bit-identical to the N-step loop. (At low resolution ~40% of the
 // aux searches settle into 2-cycles, which would otherwise run all N steps.)
-// Returns the cam_ff sample at the final pixel in `m`.
+// Only cam_ff.xy is live during the search (8-byte gathers, fewer registers
+// across the lock-step chains); callers fetch the full record at the end.
 __device__ __forceinline__ uint32_t pack_px(uint32_t x, uint32_t y) { return (y << 16) | x; }
 
 // One fixed-point search in flight. All searches of a pixel (fwd/bwd
@@ -85,7 +86,7 @@ __device__ __forceinline__ uint32_t pack_px(uint32_t x, uint32_t y) { return (y 
 // dependent gathers overlap instead of running one chain after another.
 struct Chain {
     uint32_t h0, h1, h2, h3;  // q_i, q_{i-1}, q_{i-2}, q_{i-3} (packed)
-    float4 m;                 // cam_ff at h0
+    float2 m;                 // cam_ff.xy at h0
     bool active;
 };
 constexpr uint32_t PX_NONE = 0xffffffffu;  // x, y <= 65534: never a packed pixel
@@ -97,7 +98,7 @@ __device__ __forceinline__ void chain_start(Chain &c, bool run, uint32_t x, uint
     c.h0 = run ? pack_px(x, y) : 0u;
     c.h1 = c.h2 = c.h3 = PX_NONE;
     c.active = run;
-    c.m = run ? at(v.cam_ff, v.w, x, y) : make_float4(0.f, 0.f, 0.f, 0.f);
+    c.m = run ? at_xy(v.cam_ff, v.w, x, y) : make_float2(0.f, 0.f);
 }
 
 // Step i (0-based) of n_iter: q_{i+1} = step(q_i), with the exits described above.
@@ -118,7 +119,7 @@ __device__ __forceinline__ void chain_step(Chain &c, int i, int n_iter, float2 p
         const int idx = p - 1 - (n_iter - (i + 1)) % p;  // c_r = h_{p-1-r}
         const uint32_t q = idx == 0 ? c.h0 : (idx == 1 ? c.h1 : (idx == 2 ? c.h2 : c.h3));
         c.h0 = q;
-        c.m = at(v.cam_ff, v.w, q & 0xFFFFu, q >> 16);
+        c.m = at_xy(v.cam_ff, v.w, q & 0xFFFFu, q >> 16);
         c.active = false;
         return;
     }
@@ -126,7 +127,7 @@ __device__ __forceinline__ void chain_step(Chain &c, int i, int n_iter, float2 p
     c.h2 = c.h1;
     c.h1 = c.h0;
     c.h0 = nt;
-    c.m = at(v.cam_ff, v.w, (uint32_t)t.x, (uint32_t)t.y);
+    c.m = at_xy(v.cam_ff, v.w, (uint32_t)t.x, (uint32_t)t.y);
 }
 
 // computeReprojError, renderer.cu:204-215
@@ -147,13 +148,19 @@ static __device__ uchar4 color_from_shadow_1(uint32_t vidx, int sfpa, const Reso
         if (v.sh_b != nullptr && bits != 0) {
             const int init_val = (int)(bits & SOFTSHADOW_VAL_MASK);
             const int my_val = (int)(bits >> (SOFTSHADOW_BITS * vidx) & SOFTSHADOW_VAL_MASK);
-            int min_val = (int)SOFTSHADOW_VAL_MASK, max_val = 0;
-            for (int i = 0; i < sfpa; i++) {
-                const int c = (int)(bits >> (SOFTSHADOW_BITS * (uint32_t)i) & SOFTSHADOW_VAL_MASK);
-                min_val = c < min_val ? c : min_val;
-                max_val = c > max_val ? c : max_val;
-            }
-            const float t = (float)(my_val - min_val) / (float)(max_val - min_val);
+            // min / max of the 2-bit codes of sub-frames 0..sfpa-1 (:220-224),
+            // bit-parallel: which code values occur among those fields
+            const uint32_t used = ((1u << (SOFTSHADOW_BITS * (uint32_t)sfpa)) - 1u) & 0x555555u;
+            const uint32_t lo = bits & used, hi = (bits >> 1) & used;
+            const bool has0 = (~lo & ~hi & used) != 0, has1 = (lo & ~hi) != 0, has2 = (hi & ~lo) != 0,
+                       has3 = (lo & hi) != 0;
+            const int min_val = has0 ? 0 : (has1 ? 1 : (has2 ? 2 : 3));
+            const int max_val = has3 ? 3 : (has2 ? 2 : (has1 ? 1 : 0));
+            // (my - min) / (max - min) with a quotient of small integers: RN32 of
+            // RN64(a) * RN64(1/b) is the IEEE quotient (see cs_coord); 0/0 is NaN
+            const int den = max_val - min_val;
+            const double inv = den == 1 ? 1.0 : (den == 2 ? 0.5 : (den == 3 ? 1.0 / 3.0 : __builtin_nan("")));
+            const float t = (float)((double)(my_val - min_val) * inv);
             const uchar4 cmin = min_val == init_val ? main_color : fetch(v.sh_a, v.w, v.h, px, py);
             const uchar4 cmax = max_val == init_val ? main_color : fetch(v.sh_b, v.w, v.h, px, py);
             const float omt = (float)(1.0 - (double)t);
@@ -294,12 +301,13 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
     int org_x[NC], org_y[NC], span_x[NC];
 #pragma unroll
     for (int c = 0; c < NC; c++) {
-        const uint32_t rx = a.w / a.fwd[c].w, ry = a.h / a.fwd[c].h;
+        const ResolveView &v = a.fwd[c];
         const uint32_t xl = min(x0 + TW - 1, a.w - 1), yl = min(y0 + TH - 1, a.h - 1);
-        org_x[c] = (int)(x0 / rx) - W;
-        org_y[c] = (int)(y0 / ry) - W;
-        span_x[c] = (int)(xl / rx - x0 / rx) + 1 + 2 * W;
-        const int span_y = (int)(yl / ry - y0 / ry) + 1 + 2 * W;
+        const uint32_t cx0 = div_small(x0, v.ratio_x, v.magic_x), cy0 = div_small(y0, v.ratio_y, v.magic_y);
+        org_x[c] = (int)cx0 - W;
+        org_y[c] = (int)cy0 - W;
+        span_x[c] = (int)(div_small(xl, v.ratio_x, v.magic_x) - cx0) + 1 + 2 * W;
+        const int span_y = (int)(div_small(yl, v.ratio_y, v.magic_y) - cy0) + 1 + 2 * W;
         stage_splat<SPAN>(s_splat[c][0], a.fwd[c], org_x[c], org_y[c], span_x[c], span_y);
         stage_splat<SPAN>(s_splat[c][1], a.bwd[c], org_x[c], org_y[c], span_x[c], span_y);
     }
@@ -324,8 +332,8 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
     cpcs[0] = pcs;
 #pragma unroll
     for (int c = 1; c < NC; c++) {
-        cx[c] = x / (a.w / a.fwd[c].w);
-        cy[c] = y / (a.h / a.fwd[c].h);
+        cx[c] = div_small(x, a.fwd[c].ratio_x, a.fwd[c].magic_x);
+        cy[c] = div_small(y, a.fwd[c].ratio_y, a.fwd[c].magic_y);
         cpcs[c] = make_float2(cs_coord(cx[c], a.fwd[c].inv_w), cs_coord(cy[c], a.fwd[c].inv_h));
     }
 
@@ -397,7 +405,7 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
             vis[c][d] = false;
             if (ran[c][d]) {
                 const uint32_t qx = ch[c][d].h0 & 0xFFFFu, qy = ch[c][d].h0 >> 16;
-                const float4 m = ch[c][d].m;
+                const float4 m = at(v.cam_ff, v.w, qx, qy);
                 si[c][d].x = qx;
                 si[c][d].y = qy;
                 si[c][d].depth = z_to_depth(at(v.z, v.w, qx, qy) + m.z);
@@ -437,7 +445,7 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
             vis[0][d] = false;
             if (ran[0][d]) {
                 const uint32_t qx = ch[0][d].h0 & 0xFFFFu, qy = ch[0][d].h0 >> 16;
-                const float4 m = ch[0][d].m;
+                const float4 m = at(v.cam_ff, v.w, qx, qy);
                 si[0][d].x = qx;
                 si[0][d].y = qy;
                 si[0][d].depth = z_to_depth(at(v.z, v.w, qx, qy) + m.z);

@@ -40,3 +40,66 @@ def test_pixel_division_random_large():
     for w in rng.integers(4097, 65536, 400):
         x = rng.integers(0, w, 4000)
         assert np.array_equal(_ref(x, int(w)), _fast(x, int(w))), w
+
+
+# ---------------------------------------------------------------------------
+# Further rewrites in the resolve kernels (each checked against the literal
+# operation of the reference's code).
+# ---------------------------------------------------------------------------
+def test_div_255_as_double_reciprocal():
+    """lerp_color's v / 255 (helper_common.h:22-29) as RN32(RN64(v) * RN64(1/255))
+    over random binary32 values of every magnitude (subnormal quotients
+    included), every quarter-integer in [-512, 512] and every integer."""
+    rng = np.random.default_rng(1)
+    bits = rng.integers(0, 2**31 - 2**23, 4_000_000, dtype=np.int64).astype(np.uint32)  # finite, positive
+    v = bits.view(np.float32)
+    v = np.concatenate([v, -v, np.arange(-2048, 2049, dtype=np.float32) / np.float32(4),
+                        rng.uniform(-300, 300, 2_000_000).astype(np.float32)])
+    ref = v / np.float32(255)
+    fast = (v.astype(np.float64) * (1.0 / 255.0)).astype(np.float32)
+    same = (ref == fast) | (np.isnan(ref) & np.isnan(fast))
+    assert same.all(), v[~same][:5]
+
+
+def test_div_small_magic_multiply():
+    """floor(x / r) = umulhi(x, floor(2^32 / r) + 1) for x, r < 2^16 (r > 1)."""
+    def fast(x, r):
+        m = (2**32 // r + 1) & 0xFFFFFFFF
+        return np.where(r == 1, x, (x.astype(np.uint64) * np.uint64(m)) >> np.uint64(32)).astype(np.int64)
+
+    x = np.arange(65536, dtype=np.int64)
+    for r in list(range(1, 130)) + [255, 256, 257, 1023, 4095, 32767, 32768, 65534, 65535]:
+        assert np.array_equal(fast(x, r), x // r), r
+    rng = np.random.default_rng(2)
+    for r in rng.integers(2, 65536, 200):
+        assert np.array_equal(fast(x, int(r)), x // int(r)), r
+
+
+def test_shadow_code_min_max_bit_parallel():
+    """getMinMaxShadowValues (renderer.cu:217-225): the loop over the 2-bit
+    codes of sub-frames 0..sfpa-1 against the kernels' bit-parallel form, for
+    random 24-bit fields and every sfpa 0..12, plus the quotient
+    (my - min) / (max - min) as RN64 reciprocal products (0/0 = NaN)."""
+    rng = np.random.default_rng(3)
+    bits = np.concatenate([rng.integers(0, 2**24, 200_000), np.array([0, 0xFFFFFF, 0x555555, 0xAAAAAA])])
+    for sfpa in range(13):
+        mn = np.full(bits.shape, 3)
+        mx = np.zeros(bits.shape, dtype=np.int64)
+        for i in range(sfpa):
+            c = (bits >> (2 * i)) & 3
+            mn = np.minimum(mn, c)
+            mx = np.maximum(mx, c)
+        used = ((1 << (2 * sfpa)) - 1) & 0x555555
+        lo, hi = bits & used, (bits >> 1) & used
+        has0 = (~lo & ~hi & used) != 0
+        has1, has2, has3 = (lo & ~hi) != 0, (hi & ~lo) != 0, (lo & hi) != 0
+        fmn = np.where(has0, 0, np.where(has1, 1, np.where(has2, 2, 3)))
+        fmx = np.where(has3, 3, np.where(has2, 2, np.where(has1, 1, 0)))
+        assert np.array_equal(fmn, mn) and np.array_equal(fmx, mx), sfpa
+    with np.errstate(invalid="ignore", divide="ignore"):
+        for a in range(4):
+            for b in range(4):
+                ref = np.float32(a) / np.float32(b)
+                inv = {1: 1.0, 2: 0.5, 3: 1.0 / 3.0}.get(b, np.nan)
+                fast = np.float32(np.float64(a) * inv)
+                assert ref == fast or (np.isnan(ref) and np.isnan(fast)) or (b == 0 and a > 0), (a, b)
