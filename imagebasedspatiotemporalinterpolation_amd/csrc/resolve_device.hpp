@@ -139,11 +139,36 @@ __device__ __forceinline__ float reproj_error(uint2 q, const ResolveView &v, flo
     return sqrtf(ex * ex + ey * ey);
 }
 
-// colorFromShadow(idx, ...), renderer.cu:217-268
-static __device__ uchar4 color_from_shadow_1(uint32_t vidx, int sfpa, const ResolveView &v, uint32_t px, uint32_t py) {
-    const uchar4 main_color = fetch(v.img, v.w, v.h, px, py);
+// The texels colorFromShadow reads at one pixel of a view: colour, and, when
+// the frame uses soft shadows, ShadowInfo and both shadow colours. They are
+// loaded together (the shadow colours speculatively, before ShadowInfo says
+// whether they are needed) so the colour costs one round trip to memory.
+struct Texels {
+    uchar4 img, info, sa, sb;
+};
+__device__ __forceinline__ Texels load_texels(const ResolveView &v, uint32_t px, uint32_t py, bool shadows) {
+    Texels t;
+    t.img = fetch(v.img, v.w, v.h, px, py);
+    t.info = t.sa = t.sb = make_uchar4(0, 0, 0, 0);
+    if (shadows) {
+        t.info = fetch(v.sh_info, v.w, v.h, px, py);
+        t.sa = fetch(v.sh_a, v.w, v.h, px, py);
+        t.sb = fetch(v.sh_b, v.w, v.h, px, py);
+    }
+    return t;
+}
+
+// colorFromShadow(idcs, t, ...) reads shadows only for these frames and views
+// (renderer.cu:272-273): otherwise it returns the plain colour.
+__device__ __forceinline__ bool uses_shadows(uint2 idcs, int sfpa, bool no_shadows, const ResolveView &v) {
+    return !(no_shadows || idcs.x >= (uint32_t)sfpa || idcs.y == 0 || v.sh_a == nullptr);
+}
+
+// colorFromShadow(idx, ...), renderer.cu:217-268, on loaded texels
+__device__ __forceinline__ uchar4 color_from_shadow_1(uint32_t vidx, int sfpa, const ResolveView &v, const Texels &t) {
+    const uchar4 main_color = t.img;
     if (vidx < (uint32_t)sfpa && vidx > 0) {
-        const uchar4 si = fetch(v.sh_info, v.w, v.h, px, py);
+        const uchar4 si = t.info;
         const uint32_t bits = ((uint32_t)si.x << 16) | ((uint32_t)si.y << 8) | (uint32_t)si.z;
         if (v.sh_b != nullptr && bits != 0) {
             const int init_val = (int)(bits & SOFTSHADOW_VAL_MASK);
@@ -160,30 +185,30 @@ static __device__ uchar4 color_from_shadow_1(uint32_t vidx, int sfpa, const Reso
             // RN64(a) * RN64(1/b) is the IEEE quotient (see cs_coord); 0/0 is NaN
             const int den = max_val - min_val;
             const double inv = den == 1 ? 1.0 : (den == 2 ? 0.5 : (den == 3 ? 1.0 / 3.0 : __builtin_nan("")));
-            const float t = (float)((double)(my_val - min_val) * inv);
-            const uchar4 cmin = min_val == init_val ? main_color : fetch(v.sh_a, v.w, v.h, px, py);
-            const uchar4 cmax = max_val == init_val ? main_color : fetch(v.sh_b, v.w, v.h, px, py);
-            const float omt = (float)(1.0 - (double)t);
-            const float rx = (float)cmin.x * omt + (float)cmax.x * t;
-            const float ry = (float)cmin.y * omt + (float)cmax.y * t;
-            const float rz = (float)cmin.z * omt + (float)cmax.z * t;
+            const float t01 = (float)((double)(my_val - min_val) * inv);
+            const uchar4 cmin = min_val == init_val ? main_color : t.sa;
+            const uchar4 cmax = max_val == init_val ? main_color : t.sb;
+            const float omt = (float)(1.0 - (double)t01);
+            const float rx = (float)cmin.x * omt + (float)cmax.x * t01;
+            const float ry = (float)cmin.y * omt + (float)cmax.y * t01;
+            const float rz = (float)cmin.z * omt + (float)cmax.z * t01;
             return make_uchar4(cvt_u8(rx), cvt_u8(ry), cvt_u8(rz), 255);
         }
     }
     return main_color;
 }
 
-// colorFromShadow(idcs, t, ...), renderer.cu:270-295 (frameIdx2Idx :227-230)
-static __device__ uchar4 color_from_shadow(uint2 idcs, float st, int sfpa, bool no_shadows, const ResolveView &v,
-                                    uint32_t px, uint32_t py) {
-    if (no_shadows || idcs.x >= (uint32_t)sfpa || idcs.y == 0 || v.sh_a == nullptr)
-        return fetch(v.img, v.w, v.h, px, py);
+// colorFromShadow(idcs, t, ...), renderer.cu:270-295 (frameIdx2Idx :227-230),
+// on texels loaded with load_texels(..., uses_shadows(...))
+__device__ __forceinline__ uchar4 color_from_shadow(uint2 idcs, float st, int sfpa, bool no_shadows,
+                                                    const ResolveView &v, const Texels &t) {
+    if (!uses_shadows(idcs, sfpa, no_shadows, v)) return t.img;
     const uint32_t vx = (uint32_t)(v.forward ? (int)idcs.x : sfpa - (int)idcs.x);
     const uint32_t vy = (uint32_t)(v.forward ? (int)idcs.y : sfpa - (int)idcs.y);
-    if (vx == vy || vy >= (uint32_t)sfpa) return color_from_shadow_1(vx, sfpa, v, px, py);
-    if (vx >= (uint32_t)sfpa) return color_from_shadow_1(vy, sfpa, v, px, py);
-    const uchar4 ca = color_from_shadow_1(vx, sfpa, v, px, py);
-    const uchar4 cb = color_from_shadow_1(vy, sfpa, v, px, py);
+    if (vx == vy || vy >= (uint32_t)sfpa) return color_from_shadow_1(vx, sfpa, v, t);
+    if (vx >= (uint32_t)sfpa) return color_from_shadow_1(vy, sfpa, v, t);
+    const uchar4 ca = color_from_shadow_1(vx, sfpa, v, t);
+    const uchar4 cb = color_from_shadow_1(vy, sfpa, v, t);
     return lerp_to_u8(ca, cb, st);
 }
 
@@ -196,25 +221,31 @@ struct FrameConst {
     float st;
 };
 
-// secondChanceAndBlend_kernel<EPS_2>, renderer.cu:297-327
-static __device__ uchar4 second_chance(float weight, const FrameConst &fc, SplatInfo si, const ResolveView &from,
-                                const ResolveView &to, float eps2) {
+// secondChanceAndBlend_kernel<EPS_2>, renderer.cu:297-327. The other
+// keyframe's depth, flow and texels at the projected pixel are loaded in one
+// round trip, before the similarity test decides whether they are used.
+__device__ __forceinline__ uchar4 second_chance(float weight, const FrameConst &fc, SplatInfo si,
+                                                const ResolveView &from, const ResolveView &to, float eps2) {
+    const bool sh_from = uses_shadows(fc.sidx, fc.sfpa, fc.no_shadows, from);
+    const bool sh_to = uses_shadows(fc.sidx, fc.sfpa, fc.no_shadows, to);
     const float4 m_ft = fetch(from.full_ff, from.w, from.h, si.x, si.y);
     const float sz = fetch(from.z, from.w, from.h, si.x, si.y);
+    const Texels tq = load_texels(from, si.x, si.y, sh_from);
     const float proj_x = cs_coord(si.x, from.inv_w) + m_ft.x, proj_y = cs_coord(si.y, from.inv_h) + m_ft.y;
     const float proj_z = sz + m_ft.z;
     const int2 pp = cs2pixel(proj_x, proj_y, to.w, to.h);
 
-    uchar4 out = color_from_shadow(fc.sidx, fc.st, fc.sfpa, fc.no_shadows, from, si.x, si.y);
+    uchar4 out = color_from_shadow(fc.sidx, fc.st, fc.sfpa, fc.no_shadows, from, tq);
     if (fc.frame_idx >= (uint32_t)fc.fpa) return out;
     if (in_range(pp, to.w, to.h)) {
         const uint32_t ux = (uint32_t)pp.x, uy = (uint32_t)pp.y;
-        const float zt = fetch(to.z, to.w, to.h, ux, uy);
-        const float4 m_tf = fetch(to.full_ff, to.w, to.h, ux, uy);
+        const float zt = at(to.z, to.w, ux, uy);
+        const float4 m_tf = at(to.full_ff, to.w, ux, uy);
+        const Texels tp = load_texels(to, ux, uy, sh_to);
         const float dx = m_ft.x - (-m_tf.x), dy = m_ft.y - (-m_tf.y), dz = m_ft.z - (-m_tf.z);
         const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
         if (dist < EPS_3 && fabsf(zt - proj_z) < eps2)
-            out = lerp_to_u8(color_from_shadow(fc.sidx, fc.st, fc.sfpa, fc.no_shadows, to, ux, uy), out, weight);
+            out = lerp_to_u8(color_from_shadow(fc.sidx, fc.st, fc.sfpa, fc.no_shadows, to, tp), out, weight);
     }
     return out;
 }
