@@ -252,29 +252,50 @@ __device__ __forceinline__ KernArg<T> *as_const(const T *p) {
 #define ISTI_CONST_TABLE(t, tg) const BatchTable *t = tg  // host pass only parses the body
 #endif
 
+// A block's work items increase by a fixed stride, so their owners (jobs or
+// requests, in prefix order) only move forward: the owner is found once by
+// binary search and then advanced, and its descriptor is re-read only when
+// it changes (each lookup is a dependent scalar load).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define ISTI_CONST_REF(T, name, ptr) const KernArg<T> &name = *as_const(ptr)
+#else
+#define ISTI_CONST_REF(T, name, ptr) const T &name = *(ptr)
+#endif
+
 __global__ __launch_bounds__(256) void clear_batch_kernel(const BatchTable *__restrict__ tg) {
     ISTI_CONST_TABLE(t, tg);
     const XcdRange r = xcd_range(t->clear_blocks);
+    if (r.first >= r.end) return;
+    const auto *begin = as_const(t->clear_begin);
+    uint32_t j = find_owner(t->clear_begin, t->n_jobs, r.first);
+    uint32_t jb = begin[j], je = begin[j + 1];
     for (uint32_t k = r.first; k < r.end; k += r.stride) {
-        const uint32_t j = find_owner(t->clear_begin, t->n_jobs, k);
-        clear_pairs(t->jobs[j].splat, t->jobs[j].n_px, (k - t->clear_begin[j]) * 256u + threadIdx.x);
+        while (k >= je) {
+            ++j;
+            jb = je;
+            je = begin[j + 1];
+        }
+        ISTI_CONST_REF(SplatJob, J, &t->jobs[j]);
+        clear_pairs(J.splat, J.n_px, (k - jb) * 256u + threadIdx.x);
     }
 }
 
 __global__ __launch_bounds__(256) void splat_batch_kernel(const BatchTable *__restrict__ tg) {
     ISTI_CONST_TABLE(t, tg);
     const XcdRange r = xcd_range(t->splat_blocks);
+    if (r.first >= r.end) return;
+    const auto *begin = as_const(t->splat_begin);
+    uint32_t j = find_owner(t->splat_begin, t->n_jobs, r.first);
+    uint32_t jb = begin[j], je = begin[j + 1];
     for (uint32_t k = r.first; k < r.end; k += r.stride) {
-        const uint32_t j = find_owner(t->splat_begin, t->n_jobs, k);
-#if defined(__HIP_DEVICE_COMPILE__)
-        const KernArg<SplatJob> &J = *as_const(&t->jobs[j]);
-        const KernArg<BatchRequest> &R = *as_const(&t->req[J.req]);
-#else
-        const SplatJob &J = t->jobs[j];
-        const BatchRequest &R = t->req[J.req];
-#endif
-        splat_pixel(J, R.vp_cam, R.vp_main[J.main_slot], R.init_linear != 0,
-                    (k - t->splat_begin[j]) * SPLAT_BLOCK + threadIdx.x);
+        while (k >= je) {
+            ++j;
+            jb = je;
+            je = begin[j + 1];
+        }
+        ISTI_CONST_REF(SplatJob, J, &t->jobs[j]);
+        ISTI_CONST_REF(BatchRequest, R, &t->req[J.req]);
+        splat_pixel(J, R.vp_cam, R.vp_main[J.main_slot], R.init_linear != 0, (k - jb) * SPLAT_BLOCK + threadIdx.x);
     }
 }
 
@@ -285,14 +306,18 @@ __global__ __launch_bounds__(256) void resolve_batch_kernel(const BatchTable *__
     const uint32_t r0 = t->var_req_begin[variant], r1 = t->var_req_begin[variant + 1];
     const uint32_t base = t->tile_begin[r0];
     const XcdRange r = xcd_range(t->tile_begin[r1] - base);
+    if (r.first >= r.end) return;
+    const auto *tile_begin = as_const(t->tile_begin);
+    uint32_t q = r0 + find_owner(t->tile_begin + r0, r1 - r0, base + r.first);
+    uint32_t qe = tile_begin[q + 1];
     for (uint32_t k = r.first; k < r.end; k += r.stride) {
-        const uint32_t q = r0 + find_owner(t->tile_begin + r0, r1 - r0, base + k);
+        while (base + k >= qe) qe = tile_begin[++q + 1];
 #if defined(__HIP_DEVICE_COMPILE__)
         const KernArg<ResolveArgs> &a = *as_const(&t->req[q].args);
 #else
         const ResolveArgs &a = t->req[q].args;
 #endif
-        resolve_tile<W, MAV, NAUX>(a, base + k - t->tile_begin[q], lds);
+        resolve_tile<W, MAV, NAUX>(a, base + k - tile_begin[q], lds);
         __syncthreads();  // the next tile restages LDS
     }
 }
