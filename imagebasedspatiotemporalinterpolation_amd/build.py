@@ -3,6 +3,7 @@
 Produces, next to this file:
   libisti.so        HIP kernels for gfx950 + the C ABI of include/isti.h
   libisti_scene.so  seeded synthetic keyframe generator (host C++, OpenMP)
+  libisti_io.so     reader of the reference's keyframe recordings (host C++, zlib)
   splinter          the host driver executable (C++, links both libraries)
 
 Run as ``python -m imagebasedspatiotemporalinterpolation_amd.build``. Every
@@ -44,6 +45,7 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-W
 
 LIB = PKG / "libisti.so"
 SCENE_LIB = PKG / "libisti_scene.so"
+IO_LIB = PKG / "libisti_io.so"
 SPLINTER = PKG / "splinter"
 
 
@@ -77,6 +79,10 @@ def build_native(verbose: bool = False) -> None:
     tmp = SCENE_LIB.with_suffix(".so.tmp")
     _run([CXX, *HOST_FLAGS, "-shared", f"-I{INCLUDE}", CSRC / "scene.cpp", "-o", tmp], verbose)
     os.replace(tmp, SCENE_LIB)
+
+    tmp = IO_LIB.with_suffix(".so.tmp")
+    _run([CXX, *HOST_FLAGS, "-shared", f"-I{INCLUDE}", CSRC / "keyframe_io.cpp", "-lz", "-o", tmp], verbose)
+    os.replace(tmp, IO_LIB)
 
     tmp = SPLINTER.with_name("splinter.tmp")
     _run([HIPCC, "-O2", "-std=c++17", "-fopenmp", f"-I{INCLUDE}", CSRC / "splinter.cpp",

@@ -92,7 +92,9 @@ struct Camera {
     D3 pos, right, up, fwd;
     double tan_half, aspect;
     uint32_t w, h;
-    double vp[16];   // column-major
+    double view[16];  // world -> camera, column-major
+    double proj[16];  // OpenGL projection (y flipped), column-major
+    double vp[16];    // proj * view
     double ivp[16];
 };
 
@@ -171,6 +173,8 @@ Camera make_camera(const Scene &sc, double t, int view) {
             cam.vp[col * 4 + row] = s;
         }
     invert4(cam.vp, cam.ivp);
+    std::memcpy(cam.view, V, sizeof(V));
+    std::memcpy(cam.proj, P, sizeof(P));
     return cam;
 }
 
@@ -392,6 +396,18 @@ extern "C" int isti_scene_camera(const isti_scene_config *cfg, double t, int vie
     }
     if (width) *width = cam.w;
     if (height) *height = cam.h;
+    return 0;
+}
+
+extern "C" int isti_scene_camera_parts(const isti_scene_config *cfg, double t, int view, float view_m[16],
+                                       float proj[16]) {
+    if (!cfg || !valid_view(cfg, view)) return -1;
+    const Scene sc = make_scene(*cfg);
+    const Camera cam = make_camera(sc, t, view);
+    for (int i = 0; i < 16; i++) {
+        if (view_m) view_m[i] = (float)cam.view[i];
+        if (proj) proj[i] = (float)cam.proj[i];
+    }
     return 0;
 }
 

@@ -69,6 +69,8 @@ def lib() -> ctypes.CDLL:
         L.isti_scene_camera.argtypes = [POINTER(SceneConfig), c_double, c_int, c_void_p, c_void_p,
                                         POINTER(c_uint32), POINTER(c_uint32)]
         L.isti_scene_camera.restype = c_int
+        L.isti_scene_camera_parts.argtypes = [POINTER(SceneConfig), c_double, c_int, c_void_p, c_void_p]
+        L.isti_scene_camera_parts.restype = c_int
         L.isti_scene_keyframe.argtypes = [POINTER(SceneConfig), c_int, c_int, c_int] + [c_void_p] * 10 + [c_int]
         L.isti_scene_keyframe.restype = c_int
         L.isti_scene_ground_truth.argtypes = [POINTER(SceneConfig), c_double, c_int, c_void_p, c_int]
@@ -102,6 +104,15 @@ def camera(cfg: SceneConfig, t: float, view: int = 0):
                                ctypes.byref(w), ctypes.byref(h)) != 0:
         raise ValueError("invalid view")
     return vp, ivp, w.value, h.value
+
+
+def camera_parts(cfg: SceneConfig, t: float, view: int = 0):
+    """(view[16], proj[16]) of `view` at time t, column-major; viewproj = proj * view."""
+    vm = np.zeros(16, np.float32)
+    pm = np.zeros(16, np.float32)
+    if lib().isti_scene_camera_parts(ctypes.byref(cfg), float(t), view, vm.ctypes.data, pm.ctypes.data) != 0:
+        raise ValueError("invalid view")
+    return vm, pm
 
 
 def keyframe(cfg: SceneConfig, k: int, view: int = 0, direction: int = 1, threads: int = 0) -> dict:

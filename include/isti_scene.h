@@ -67,6 +67,10 @@ int isti_scene_defaults(int workload, uint64_t seed, isti_scene_config *cfg);
 int isti_scene_camera(const isti_scene_config *cfg, double t, int view, float viewproj[16],
                       float inv_viewproj[16], uint32_t *width, uint32_t *height);
 
+/* The view (world -> camera) and projection matrices of `view` at time t,
+ * column-major; viewproj of isti_scene_camera is proj * view. */
+int isti_scene_camera_parts(const isti_scene_config *cfg, double t, int view, float view_m[16], float proj[16]);
+
 /* Keyframe `k` of view `view`. direction +1: motion (and shadow sub-frames)
  * toward keyframe k+1 (the reference's forward flow field / forward shadow
  * dirs); -1: toward keyframe k-1 (backward). Outputs (any may be NULL):

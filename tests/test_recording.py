@@ -1,0 +1,169 @@
+"""Reader of the reference's keyframe recordings (libisti_io, include/isti_io.h)
+against the package's own writers and against closed forms: PNG filters and
+channel counts, OpenEXR compressions and pixel types (every half value),
+cam.json layout and defaults, projection normalisation, and a whole exported
+synthetic recording read back."""
+import json
+
+import numpy as np
+import pytest
+
+from imagebasedspatiotemporalinterpolation_amd import recording as RC
+from imagebasedspatiotemporalinterpolation_amd import scene as S
+
+
+@pytest.mark.parametrize("channels", [1, 2, 3, 4])
+@pytest.mark.parametrize("filt", [0, 1, 2, 3, 4, None])
+def test_png_channels_and_filters(tmp_path, channels, filt):
+    rng = np.random.default_rng(channels * 10 + (filt or 7))
+    img = rng.integers(0, 256, (23, 41, channels), dtype=np.uint8)
+    img[5:9] = 200  # runs where the filters' predictions matter
+    p = tmp_path / "x.png"
+    p.write_bytes(RC.encode_png(img, None if filt is None else (filt,)))
+    got = RC.decode_png(p)
+    exp = np.zeros((23, 41, 4), np.uint8)
+    if channels in (1, 2):
+        exp[..., :3] = img[..., :1]
+        exp[..., 3] = img[..., 1] if channels == 2 else 255
+    else:
+        exp[..., :channels] = img
+        if channels == 3:
+            exp[..., 3] = 255
+    assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("compression", ["none", "zips", "zip"])
+@pytest.mark.parametrize("pixel_type", ["half", "float"])
+def test_exr_compressions_and_types(tmp_path, compression, pixel_type):
+    rng = np.random.default_rng(3)
+    h, w = 37, 29  # ZIP chunks of 16 lines with a ragged last one
+    ch = {n: (rng.standard_normal((h, w)) * 10).astype(np.float32) for n in ("B", "R", "G", "Z")}
+    p = tmp_path / "x.exr"
+    p.write_bytes(RC.encode_exr(ch, compression, pixel_type))
+    got = RC.decode_exr(p, ("R", "G", "B", "A", "Z"))
+    cast = (lambda a: a.astype(np.float16).astype(np.float32)) if pixel_type == "half" else (lambda a: a)
+    exp = np.stack([cast(ch["R"]), cast(ch["G"]), cast(ch["B"]), np.zeros((h, w), np.float32), cast(ch["Z"])], -1)
+    assert np.array_equal(got, exp)
+
+
+def test_exr_every_half_value(tmp_path):
+    bits = np.arange(65536, dtype=np.uint16)
+    halves = bits.view(np.float16).reshape(256, 256)
+    p = tmp_path / "h.exr"
+    p.write_bytes(RC.encode_exr({"R": halves.astype(np.float32)}, "zip", "half"))
+    got = RC.decode_exr(p, ("R",))[..., 0]
+    exp = halves.astype(np.float32)
+    same = (got == exp) | (np.isnan(got) & np.isnan(exp))
+    assert same.all()
+    # sign of zeros and infinities survive
+    assert np.array_equal(np.signbit(got), np.signbit(exp))
+
+
+def test_decoders_reject_what_they_do_not_read(tmp_path):
+    p = tmp_path / "bad.png"
+    p.write_bytes(b"not a png at all")
+    with pytest.raises(IOError, match="not a PNG"):
+        RC.decode_png(p)
+    with pytest.raises(IOError, match="cannot read"):
+        RC.decode_exr(tmp_path / "missing.exr")
+
+
+def _cam_json(path, extra=None):
+    frames = []
+    for k in range(3):
+        m = {f"e{r}{c}": float(100 * k + 10 * r + c) for r in range(4) for c in range(4)}
+        frames.append({"idx": 7 + k, "w2c": m, "proj": m})
+    d = {"resolution": {"x": 64, "y": 32}, "frames": frames}
+    d.update(extra or {})
+    path.mkdir(parents=True, exist_ok=True)
+    (path / "cam.json").write_text(json.dumps(d))
+
+
+def test_cam_json_layout_defaults_and_priority(tmp_path):
+    _cam_json(tmp_path / "a")
+    info, frames = RC.read_camera(tmp_path / "a")
+    assert (info["width"], info["height"], info["n_frames"]) == (64, 32, 3)
+    # defaults of FrameFileReader::initCamInfo (file_reader.cu:96-104)
+    assert (info["client_fps"], info["server_fps"], info["shadow_fps"], info["priority"]) == (60, 10, 60, 3)
+    idx, w2c, proj = frames[1]
+    assert idx == 8
+    for r in range(4):
+        for c in range(4):  # "eRC" is row R, column C; glm is column-major
+            assert w2c[c * 4 + r] == 100 + 10 * r + c
+    _cam_json(tmp_path / "b", {"clientFps": 90, "serverFps": 15, "objectMotion": False, "shadowGradients": True})
+    info, _ = RC.read_camera(tmp_path / "b")
+    assert (info["client_fps"], info["server_fps"], info["priority"]) == (90, 15, 1)
+
+
+def test_projection_normalisation_closed_form():
+    """normalizeProjMat (file_reader.cu:219-230) maps a reversed-Z [0, 1]
+    depth to OpenGL [-1, 1] and flips y: z_gl = 2 (1 - d) - 1 for the same
+    clip w, y_gl = -y."""
+    x = np.array([0.3, -0.2, 0.7, 1.0])  # clip-space point of the raw projection (d = z / w = 0.7)
+    y = RC.NORMALIZE @ x
+    assert np.allclose(y, [0.3, 0.2, 2 * (1 - 0.7) - 1, 1.0])
+
+
+def test_exported_recording_reads_back(tmp_path):
+    cfg = S.defaults(2, 4)
+    cfg.width, cfg.height = 240, 136
+    cfg.n_aux = 0
+    cam_dir = RC.write_recording(tmp_path, cfg, 3, exr_compression="zip")
+    info, frames = RC.read_camera(cam_dir)
+    assert (info["width"], info["height"], info["n_frames"]) == (240, 136, 3)
+    for pair, which in ((0, 0), (0, 1), (1, 0), (1, 1)):
+        k = pair + which
+        ref = S.keyframe(cfg, k, 0, -1 if which else +1)
+        got = RC.read_view(cam_dir, pair, which)
+        assert np.array_equal(got["color"], ref["color"])
+        assert np.array_equal(got["motion"], ref["motion"])
+        for key in ("shadow_info", "shadow_color_a", "shadow_color_b"):
+            assert np.array_equal(got[key], ref[key]), key
+        # depth is stored as reversed-Z [0, 1]; the reader applies
+        # convertDepthToZ (file_reader.cu:17-37) in single precision
+        d = RC.z_to_file_depth(ref["z"])
+        exp_z = np.float32(2.0) * (np.float32(1.0) - d) - np.float32(1.0)
+        assert np.array_equal(got["z"], exp_z.astype(np.float32))
+        assert np.max(np.abs(got["z"] - ref["z"])) < 1e-6
+        # normalised projection * view reproduces the scene's camera
+        assert np.allclose(got["viewproj"], ref["viewproj"], rtol=1e-5, atol=1e-6)
+        vp = got["viewproj"].reshape(4, 4).T.astype(np.float64)
+        ivp = got["inv_viewproj"].reshape(4, 4).T.astype(np.float64)
+        assert np.allclose(vp @ ivp, np.eye(4), atol=1e-5)
+    with pytest.raises(IOError, match="beyond"):
+        RC.read_view(cam_dir, 2, 0)
+
+
+def _rot_y(deg, tx=0.0, ty=0.0, tz=0.0):
+    a = np.radians(deg)
+    m = np.eye(4)
+    m[0, 0], m[0, 2], m[2, 0], m[2, 2] = np.cos(a), np.sin(a), -np.sin(a), np.cos(a)
+    m[:3, 3] = (tx, ty, tz)
+    return m.T.reshape(16).astype(np.float32)  # column-major
+
+
+def test_camera_interpolation_properties():
+    """glm::interpolate of view matrices (src/main.cu:26-29): the ends give the
+    keyframe views, a yaw halves, translations lerp, the 180-degree branch
+    keeps an orthonormal rotation, and viewproj is proj * view_t."""
+    P = np.eye(4, dtype=np.float32).reshape(16)
+    a, b = _rot_y(10, 1, 2, 3), _rot_y(50, 3, 2, -1)
+    v0, vp0, _ = RC.camera_at(a, b, P, 0.0)
+    assert np.array_equal(v0, a) and np.array_equal(vp0, a)
+    v1, _, _ = RC.camera_at(a, b, P, 1.0)
+    assert np.allclose(v1, b, atol=2e-6)
+    vh, _, ivh = RC.camera_at(a, b, P, 0.5)
+    assert np.allclose(vh, _rot_y(30, 2, 2, 1), atol=2e-6)
+    assert np.allclose(vh.reshape(4, 4).T @ ivh.reshape(4, 4).T, np.eye(4), atol=1e-5)
+    # same rotation: only the translation moves
+    vt, _, _ = RC.camera_at(_rot_y(20, 0, 0, 0), _rot_y(20, 4, 0, 0), P, 0.25)
+    assert np.allclose(vt, _rot_y(20, 1, 0, 0), atol=2e-6)
+    # half-turn: the near-symmetric branch; the midpoint is a quarter turn about y
+    vq, _, _ = RC.camera_at(_rot_y(0), _rot_y(180), P, 0.5)
+    R = vq.reshape(4, 4).T[:3, :3]
+    assert np.allclose(R @ R.T, np.eye(3), atol=1e-5)
+    assert np.allclose(np.abs(R), np.abs(_rot_y(90).reshape(4, 4).T[:3, :3]), atol=1e-5)
+    # a projection multiplies on the left
+    Pm = np.diag([2.0, -1.0, 0.5, 1.0]).astype(np.float32).T.reshape(16)
+    _, vpp, _ = RC.camera_at(a, b, Pm, 0.5)
+    assert np.allclose(vpp.reshape(4, 4).T, np.diag([2.0, -1.0, 0.5, 1.0]) @ vh.reshape(4, 4).T, atol=1e-6)
