@@ -4,7 +4,7 @@ Produces, next to this file:
   libisti.so        HIP kernels for gfx950 + the C ABI of include/isti.h
   libisti_scene.so  seeded synthetic keyframe generator (host C++, OpenMP)
   libisti_io.so     reader of the reference's keyframe recordings (host C++, zlib)
-  splinter          the host driver executable (C++, links both libraries)
+  splinter          the host driver executable (C++, links the three libraries)
 
 Run as ``python -m imagebasedspatiotemporalinterpolation_amd.build``. Every
 call recompiles (the sources are small). hipcc cross-compiles gfx950 code
@@ -86,7 +86,7 @@ def build_native(verbose: bool = False) -> None:
 
     tmp = SPLINTER.with_name("splinter.tmp")
     _run([HIPCC, "-O2", "-std=c++17", "-fopenmp", f"-I{INCLUDE}", CSRC / "splinter.cpp",
-          f"-L{PKG}", "-listi", "-listi_scene", f"-Wl,-rpath,$ORIGIN", "-o", tmp], verbose)
+          f"-L{PKG}", "-listi", "-listi_scene", "-listi_io", f"-Wl,-rpath,$ORIGIN", "-o", tmp], verbose)
     os.replace(tmp, SPLINTER)
 
 

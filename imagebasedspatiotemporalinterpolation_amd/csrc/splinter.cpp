@@ -4,14 +4,17 @@
 //   splinter <input> <output-dir> [-w] [-r W H] [-a NAME...] [-l N] [-m 0..4]
 //            [-lh] [-ns] [-hb] [--frames N] [--iters N] [--seed S]
 //
-// <input> selects the keyframe source. The reference reads Unity recordings
-// from a directory (src/file_reader.cu); this driver takes a seeded
-// synthetic scene instead: "scene:1", "scene:2" or "scene:3" (optionally
-// "scene:2:SEED"). -a NAME... enables aux cameras (one per name, at most 2
-// with the synthetic scene: left/right). The target camera follows the
-// scene's analytic camera path at client-frame times (the reference's -c
-// camera path). -w writes <output-dir>/NNNN.png per frame. Per-frame GPU
-// times (hipEvents) are summarised at the end as Mpix/s and p50/p99 ms.
+// <input> is either a keyframe recording in the reference's on-disk format
+// (a directory with main/ and one directory per -a NAME; read through
+// include/isti_io.h, as FrameFileReader does) or a seeded synthetic scene:
+// "scene:1", "scene:2" or "scene:3" (optionally "scene:2:SEED"; -a names
+// select up to 2 aux cameras, left/right). With a recording the client runs
+// at 120 Hz, frames_per_anchor = 120 / serverFps and the target camera is the
+// interpolation of the keyframe views (main.cu:139-161); with a synthetic
+// scene the target camera follows the scene's analytic camera path at
+// client-frame times (the reference's -c camera path). -w writes
+// <output-dir>/NNNN.png per frame. Per-frame GPU times (hipEvents) are
+// summarised at the end as Mpix/s and p50/p99 ms.
 
 #include <hip/hip_runtime.h>
 
@@ -26,6 +29,7 @@
 #include <string>
 #include <vector>
 
+#include "isti_io.h"
 #include "isti_renderer.hpp"
 #include "isti_scene.h"
 
@@ -105,6 +109,8 @@ struct DevView {
     uint8_t *color = nullptr, *sh_info = nullptr, *sh_a = nullptr, *sh_b = nullptr;
     float *z = nullptr, *mx = nullptr, *my = nullptr, *mz = nullptr;
     float vp[16], ivp[16];
+    float view[16], proj[16];         // recordings: w2c and normalised projection
+    bool has_motion = true, has_shadows = true;
     void alloc(uint32_t W, uint32_t H, bool shadows) {
         w = W; h = H;
         const size_t n = (size_t)W * H;
@@ -124,8 +130,8 @@ struct DevView {
         std::memset(&d, 0, sizeof(d));
         d.width = w; d.height = h;
         d.color = color; d.z = z;
-        d.motion_x = mx; d.motion_y = my; d.motion_z = mz;
-        d.shadow_info = sh_info; d.shadow_color_a = sh_a; d.shadow_color_b = sh_b;
+        if (has_motion) { d.motion_x = mx; d.motion_y = my; d.motion_z = mz; }
+        if (has_shadows) { d.shadow_info = sh_info; d.shadow_color_a = sh_a; d.shadow_color_b = sh_b; }
         std::memcpy(d.viewproj, vp, sizeof(vp));
         std::memcpy(d.inv_viewproj, ivp, sizeof(ivp));
         d.priority = priority;
@@ -157,19 +163,53 @@ void load_keyframe(const isti_scene_config &cfg, int k, int view, int dir, DevVi
     }
 }
 
+// Reads keyframe view `which` of pair `pair` of a recorded camera and uploads it.
+void load_recorded(const std::string &cam_dir, int pair, int which, DevView &dv) {
+    isti_io_view v;
+    if (isti_io_read_view(cam_dir.c_str(), pair, which, &v) != ISTI_OK) {
+        std::fprintf(stderr, "%s\n", isti_io_last_error());
+        std::exit(2);
+    }
+    if (v.width != dv.w || v.height != dv.h) {
+        std::fprintf(stderr, "%s: keyframe size changed within the recording\n", cam_dir.c_str());
+        std::exit(2);
+    }
+    const size_t n = (size_t)dv.w * dv.h;
+    HIP_OK(hipMemcpy(dv.color, v.color, n * 4, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(dv.z, v.z, n * 4, hipMemcpyHostToDevice));
+    dv.has_motion = v.motion_x != nullptr;
+    if (dv.has_motion) {
+        HIP_OK(hipMemcpy(dv.mx, v.motion_x, n * 4, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(dv.my, v.motion_y, n * 4, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(dv.mz, v.motion_z, n * 4, hipMemcpyHostToDevice));
+    }
+    dv.has_shadows = v.shadow_info && v.shadow_color_a && v.shadow_color_b && dv.sh_info;
+    if (dv.has_shadows) {
+        HIP_OK(hipMemcpy(dv.sh_info, v.shadow_info, n * 4, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(dv.sh_a, v.shadow_color_a, n * 4, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(dv.sh_b, v.shadow_color_b, n * 4, hipMemcpyHostToDevice));
+    }
+    std::memcpy(dv.vp, v.viewproj, sizeof(dv.vp));
+    std::memcpy(dv.ivp, v.inv_viewproj, sizeof(dv.ivp));
+    std::memcpy(dv.view, v.view, sizeof(dv.view));
+    std::memcpy(dv.proj, v.proj, sizeof(dv.proj));
+    isti_io_free_view(&v);
+}
+
 void usage() {
     std::puts(
         "Splinter - Split-Rendering Frame Interpolation (MI355X)\n"
-        "usage: splinter <scene:N[:SEED]> <output-dir> [options]\n"
+        "usage: splinter <recording-dir | scene:N[:SEED]> <output-dir> [options]\n"
         "  -w, --write-images         write PNG frames to output-dir\n"
         "  -r, --resolution W H       render resolution (sets the scene's main view size)\n"
-        "  -a, --aux-dirs NAME...     aux camera names (synthetic scene: up to 2)\n"
+        "  -a, --aux-dirs NAME...     aux camera directories (synthetic scene: up to 2 names)\n"
         "  -l, --latency N            delay keyframes by N client frames\n"
         "  -m, --mode 0..4            render mode (RenderMode)\n"
         "  -lh, --latency-hold-last   hold the last frame while a keyframe is late\n"
         "  -ns, --no-shadows          disable dynamic shadows\n"
         "  -hb, --holes-black         clear the output every frame (no hole filling)\n"
-        "  --frames N                 client frames to render (default 2 keyframe pairs)\n"
+        "  --frames N                 client frames to render (default: scene 2 keyframe pairs,\n"
+        "                             recording 15 s at 120 Hz or its length)\n"
         "  --iters N                  fixed-point search cap (default: the mode's own)\n"
         "  --seed S                   scene seed");
 }
@@ -207,41 +247,82 @@ int main(int argc, char **argv) {
         else pos.push_back(a);
     }
     if (pos.size() != 2 || mode < 0 || mode > 4) { usage(); return 1; }
-    if (pos[0].rfind("scene:", 0) != 0) {
-        std::fprintf(stderr, "input must be scene:N[:SEED] (on-disk recordings are not supported)\n");
-        return 1;
-    }
-    int workload = 1;
-    uint64_t scene_seed = 1;
-    std::sscanf(pos[0].c_str() + 6, "%d", &workload);
-    const size_t c2 = pos[0].find(':', 6);
-    if (c2 != std::string::npos) scene_seed = std::strtoull(pos[0].c_str() + c2 + 1, nullptr, 10);
-    if (seed >= 0) scene_seed = (uint64_t)seed;
+    const bool synthetic = pos[0].rfind("scene:", 0) == 0;
     isti_scene_config cfg;
-    if (isti_scene_defaults(workload, scene_seed, &cfg) != 0) { std::fprintf(stderr, "unknown scene\n"); return 1; }
-    if (res_w > 0 && res_h > 0) {
-        const uint32_t ow = cfg.width, oh = cfg.height;
-        cfg.width = (uint32_t)res_w;
-        cfg.height = (uint32_t)res_h;
-        if (cfg.n_aux > 0) {  // keep the aux/main ratio
-            cfg.aux_width = cfg.width * cfg.aux_width / ow;
-            cfg.aux_height = cfg.height * cfg.aux_height / oh;
+    std::memset(&cfg, 0, sizeof(cfg));
+    uint32_t W = 0, H = 0, AW = 0, AH = 0;
+    int n_eyes = 1, n_aux = 0, fpa = 1, sfpa = 0, total = 0, n_keyframes = 0;
+    bool shadows = false;
+    std::vector<int> aux_prio;
+    std::string main_dir;
+    std::vector<std::string> aux_dirs;
+    if (synthetic) {
+        int workload = 1;
+        uint64_t scene_seed = 1;
+        std::sscanf(pos[0].c_str() + 6, "%d", &workload);
+        const size_t c2 = pos[0].find(':', 6);
+        if (c2 != std::string::npos) scene_seed = std::strtoull(pos[0].c_str() + c2 + 1, nullptr, 10);
+        if (seed >= 0) scene_seed = (uint64_t)seed;
+        if (isti_scene_defaults(workload, scene_seed, &cfg) != 0) { std::fprintf(stderr, "unknown scene\n"); return 1; }
+        if (res_w > 0 && res_h > 0) {
+            const uint32_t ow = cfg.width, oh = cfg.height;
+            cfg.width = (uint32_t)res_w;
+            cfg.height = (uint32_t)res_h;
+            if (cfg.n_aux > 0) {  // keep the aux/main ratio
+                cfg.aux_width = cfg.width * cfg.aux_width / ow;
+                cfg.aux_height = cfg.height * cfg.aux_height / oh;
+            }
         }
+        W = cfg.width;
+        H = cfg.height;
+        AW = cfg.aux_width;
+        AH = cfg.aux_height;
+        n_eyes = cfg.n_eyes;
+        n_aux = n_eyes == 2 ? 0 : std::min<int>((int)aux_names.size(), cfg.n_aux);
+        if (n_eyes == 1 && (int)aux_names.size() > cfg.n_aux)
+            std::fprintf(stderr, "note: scene has %d aux cameras; extra -a names ignored\n", cfg.n_aux);
+        fpa = cfg.keyframe_spacing;                 // main.cu:144
+        sfpa = cfg.shadows ? cfg.shadow_frames : 0;  // main.cu:145
+        total = frames > 0 ? frames : 2 * fpa;
+        shadows = cfg.shadows != 0;
+        aux_prio.assign((size_t)n_aux, 3);  // objectMotion*2 + shadowGradients
+    } else {
+        // FrameFileReader::init (file_reader.cu:109-136) and main.cu:139-145
+        main_dir = pos[0] + "/main";
+        isti_io_camera mc;
+        if (isti_io_read_camera(main_dir.c_str(), &mc, nullptr, 0) != ISTI_OK) {
+            std::fprintf(stderr, "%s\n", isti_io_last_error());
+            return 1;
+        }
+        if (mc.server_fps <= 0 || mc.n_frames < 2) { std::fprintf(stderr, "recording needs serverFps > 0 and 2 frames\n"); return 1; }
+        W = mc.width;
+        H = mc.height;
+        n_keyframes = mc.n_frames;
+        fpa = 120 / mc.server_fps;  // the fixed 120 Hz client of main.cu:139-144
+        sfpa = mc.shadow_fps / mc.server_fps;
+        n_aux = std::min<int>((int)aux_names.size(), ISTI_MAX_AUX);
+        for (int a = 0; a < n_aux; a++) {
+            aux_dirs.push_back(pos[0] + "/" + aux_names[(size_t)a]);
+            isti_io_camera ac;
+            if (isti_io_read_camera(aux_dirs.back().c_str(), &ac, nullptr, 0) != ISTI_OK) {
+                std::fprintf(stderr, "%s\n", isti_io_last_error());
+                return 1;
+            }
+            if (a > 0 && (ac.width != AW || ac.height != AH)) { std::fprintf(stderr, "aux cameras differ in size\n"); return 1; }
+            AW = ac.width;
+            AH = ac.height;
+            aux_prio.push_back(ac.priority);
+            n_keyframes = std::min(n_keyframes, ac.n_frames);
+        }
+        if (res_w > 0) std::fprintf(stderr, "note: -r is ignored for recordings (render size = main keyframe size)\n");
+        shadows = true;  // planes are used only where the recording has them
+        total = frames > 0 ? frames : std::min(120 * 15, (n_keyframes - 1) * fpa);
     }
-    const int n_eyes = cfg.n_eyes;
-    const int n_aux = n_eyes == 2 ? 0 : std::min<int>((int)aux_names.size(), cfg.n_aux);
-    if (n_eyes == 1 && (int)aux_names.size() > cfg.n_aux)
-        std::fprintf(stderr, "note: scene has %d aux cameras; extra -a names ignored\n", cfg.n_aux);
     const fs::path out_dir = pos[1];
     if (!fs::exists(out_dir) && !fs::create_directories(out_dir)) {
         std::fprintf(stderr, "Output directory could not be created\n");
         return 1;
     }
-
-    const int fpa = cfg.keyframe_spacing;                 // main.cu:144
-    const int sfpa = cfg.shadows ? cfg.shadow_frames : 0;  // main.cu:145
-    const int total = frames > 0 ? frames : 2 * fpa;
-    const bool shadows = cfg.shadows != 0;
 
     // per eye: fwd/bwd main views + aux curr/next
     struct Eye {
@@ -255,12 +336,12 @@ int main(int argc, char **argv) {
     for (int e = 0; e < n_eyes; e++) {
         Eye &E = eyes[e];
         for (int s = 0; s < 2; s++) {
-            E.slot[s].alloc(cfg.width, cfg.height, shadows);
+            E.slot[s].alloc(W, H, shadows);
             E.aux[s].resize(n_aux);
-            for (int a = 0; a < n_aux; a++) E.aux[s][a].alloc(cfg.aux_width, cfg.aux_height, shadows);
+            for (int a = 0; a < n_aux; a++) E.aux[s][a].alloc(AW, AH, shadows);
         }
-        HIP_OK(hipMalloc(&E.out, (size_t)cfg.width * cfg.height * 4));
-        HIP_OK(hipMemset(E.out, 0, (size_t)cfg.width * cfg.height * 4));
+        HIP_OK(hipMalloc(&E.out, (size_t)W * H * 4));
+        HIP_OK(hipMemset(E.out, 0, (size_t)W * H * 4));
         E.renderer = new isti::Renderer(n_aux);
         E.fd = isti::FrameData(n_aux);
     }
@@ -273,28 +354,43 @@ int main(int argc, char **argv) {
             E.fd.fwd = E.slot[fwd_slot].desc(0);
             E.fd.bwd = E.slot[1 - fwd_slot].desc(0);
             for (int a = 0; a < n_aux; a++) {
-                E.fd.aux_cam_images[a].curr = E.aux[fwd_slot][a].desc(3);
-                E.fd.aux_cam_images[a].next = E.aux[1 - fwd_slot][a].desc(3);
-                E.fd.aux_cam_images[a].priority = 3;  // objectMotion*2 + shadowGradients
+                E.fd.aux_cam_images[a].curr = E.aux[fwd_slot][a].desc(aux_prio[(size_t)a]);
+                E.fd.aux_cam_images[a].next = E.aux[1 - fwd_slot][a].desc(aux_prio[(size_t)a]);
+                E.fd.aux_cam_images[a].priority = aux_prio[(size_t)a];
             }
         }
     };
     // file_reader.cu:109-136: keyframe 0 goes into the bwd slot
     for (int e = 0; e < n_eyes; e++) {
-        load_keyframe(cfg, 0, view_of(e, 0), -1, eyes[e].slot[1 - fwd_slot], true);
-        for (int a = 0; a < n_aux; a++) load_keyframe(cfg, 0, 1 + a, -1, eyes[e].aux[1 - fwd_slot][a], true);
+        if (synthetic) {
+            load_keyframe(cfg, 0, view_of(e, 0), -1, eyes[e].slot[1 - fwd_slot], true);
+            for (int a = 0; a < n_aux; a++) load_keyframe(cfg, 0, 1 + a, -1, eyes[e].aux[1 - fwd_slot][a], true);
+        } else {
+            load_recorded(main_dir, 0, 0, eyes[e].slot[1 - fwd_slot]);
+            for (int a = 0; a < n_aux; a++) load_recorded(aux_dirs[(size_t)a], 0, 0, eyes[e].aux[1 - fwd_slot][a]);
+        }
     }
     next_key = 1;
     // file_reader.cu:167-217: swap, read the next keyframe into bwd, re-read
     // the (new) fwd keyframe's forward motion / shadow data
     auto read_next = [&]() {
         fwd_slot = 1 - fwd_slot;
+        if (!synthetic && next_key >= n_keyframes) return;  // end of the recording: swap only (:169-172)
         for (int e = 0; e < n_eyes; e++) {
-            load_keyframe(cfg, next_key - 1, view_of(e, 0), +1, eyes[e].slot[fwd_slot], false);
-            load_keyframe(cfg, next_key, view_of(e, 0), -1, eyes[e].slot[1 - fwd_slot], true);
-            for (int a = 0; a < n_aux; a++) {
-                load_keyframe(cfg, next_key - 1, 1 + a, +1, eyes[e].aux[fwd_slot][a], false);
-                load_keyframe(cfg, next_key, 1 + a, -1, eyes[e].aux[1 - fwd_slot][a], true);
+            if (synthetic) {
+                load_keyframe(cfg, next_key - 1, view_of(e, 0), +1, eyes[e].slot[fwd_slot], false);
+                load_keyframe(cfg, next_key, view_of(e, 0), -1, eyes[e].slot[1 - fwd_slot], true);
+                for (int a = 0; a < n_aux; a++) {
+                    load_keyframe(cfg, next_key - 1, 1 + a, +1, eyes[e].aux[fwd_slot][a], false);
+                    load_keyframe(cfg, next_key, 1 + a, -1, eyes[e].aux[1 - fwd_slot][a], true);
+                }
+            } else {
+                load_recorded(main_dir, next_key - 1, 0, eyes[e].slot[fwd_slot]);
+                load_recorded(main_dir, next_key - 1, 1, eyes[e].slot[1 - fwd_slot]);
+                for (int a = 0; a < n_aux; a++) {
+                    load_recorded(aux_dirs[(size_t)a], next_key - 1, 0, eyes[e].aux[fwd_slot][a]);
+                    load_recorded(aux_dirs[(size_t)a], next_key - 1, 1, eyes[e].aux[1 - fwd_slot][a]);
+                }
             }
         }
         next_key++;
@@ -304,7 +400,7 @@ int main(int argc, char **argv) {
     HIP_OK(hipEventCreate(&ev0));
     HIP_OK(hipEventCreate(&ev1));
     std::vector<float> frame_ms;
-    std::vector<uint8_t> host_img((size_t)cfg.width * cfg.height * 4);
+    std::vector<uint8_t> host_img((size_t)W * H * 4);
     for (int i = 0; i < total; i += fpa) {          // main.cu:149
         for (int b = 0; b < fpa; b++) {              // main.cu:152
             if (latency == b) read_next();
@@ -315,18 +411,25 @@ int main(int argc, char **argv) {
             float ms_total = 0.f;
             for (int e = 0; e < n_eyes; e++) {
                 Eye &E = eyes[e];
-                if (holes_black) HIP_OK(hipMemset(E.out, 0, (size_t)cfg.width * cfg.height * 4));
+                if (holes_black) HIP_OK(hipMemset(E.out, 0, (size_t)W * H * 4));
                 if (!(b < latency && hold_last)) {
                     isti::RenderInfo ri;
-                    ri.resolution[0] = cfg.width;
-                    ri.resolution[1] = cfg.height;
+                    ri.resolution[0] = W;
+                    ri.resolution[1] = H;
                     ri.alpha = t;
-                    uint32_t w, h;
-                    // client camera at absolute client-frame time
-                    isti_scene_camera(&cfg, (double)(i + b), view_of(e, 0), ri.viewproj_mat, ri.inv_viewproj_mat, &w, &h);
+                    if (synthetic) {  // client camera at absolute client-frame time
+                        uint32_t w, h;
+                        isti_scene_camera(&cfg, (double)(i + b), view_of(e, 0), ri.viewproj_mat, ri.inv_viewproj_mat, &w, &h);
+                    } else {  // getIntermediateViewMatrix (main.cu:26-29, 158-161)
+                        const DevView &F = E.slot[fwd_slot], &B = E.slot[1 - fwd_slot];
+                        if (isti_io_camera_at(F.view, B.view, F.proj, t, nullptr, ri.viewproj_mat, ri.inv_viewproj_mat) != ISTI_OK) {
+                            std::fprintf(stderr, "%s\n", isti_io_last_error());
+                            return 2;
+                        }
+                    }
                     ri.additional_info = {fpa, sfpa, no_shadows, (isti::RenderMode)mode};
                     ri.max_search_iterations = iters;
-                    isti::RenderBuffer rb{E.out, (size_t)cfg.width * 4, cfg.width, cfg.height};
+                    isti::RenderBuffer rb{E.out, (size_t)W * 4, W, H};
                     HIP_OK(hipEventRecord(ev0, nullptr));
                     try {
                         E.renderer->render(rb, ri, E.fd);
@@ -345,7 +448,7 @@ int main(int argc, char **argv) {
                     char name[64];
                     if (n_eyes == 2) std::snprintf(name, sizeof(name), "%04d_%s.png", frame_number - 1, e == 0 ? "left" : "right");
                     else std::snprintf(name, sizeof(name), "%04d.png", frame_number - 1);
-                    write_png(out_dir / name, host_img.data(), cfg.width, cfg.height);
+                    write_png(out_dir / name, host_img.data(), W, H);
                 }
             }
             frame_ms.push_back(ms_total);
@@ -360,7 +463,7 @@ int main(int argc, char **argv) {
         std::sort(s.begin(), s.end());
         double sum = 0;
         for (float v : s) sum += v;
-        const double mpix = (double)cfg.width * cfg.height * n_eyes * s.size() / (sum * 1e-3) / 1e6;
+        const double mpix = (double)W * H * n_eyes * s.size() / (sum * 1e-3) / 1e6;
         std::printf("frames %zu  mean %.3f ms  p50 %.3f ms  p99 %.3f ms  %.1f Mpix/s (GPU time per frame)\n",
                     s.size(), sum / s.size(), s[s.size() / 2], s[std::min(s.size() - 1, (size_t)(0.99 * s.size()))], mpix);
     }
