@@ -22,6 +22,8 @@ struct Mat4 {
 };
 
 // src/renderer.cu:16-22, 27-28
+// Search steps granted to a learned start (isti_render_learned).
+constexpr int LEARNED_STEPS = 4;
 constexpr float EPS_1_FULL_RES = 0.002f;
 constexpr float EPS_1_LOW_RES = EPS_1_FULL_RES * 4.0f;
 constexpr float EPS_2_FULL_RES = 0.001f;

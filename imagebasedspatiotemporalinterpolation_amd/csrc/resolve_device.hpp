@@ -412,7 +412,16 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
         chain_start(ch[c][1], ran[c][1], si[c][1].x, si[c][1].y, a.bwd[c]);
     }
     uint32_t steps = 0;  // main-camera search steps taken (learned statistics)
+    // a learned start gets at most LEARNED_STEPS steps: one still moving by
+    // then is judged at that point (and falls back to the window start if
+    // rejected), so no wave waits for a wandering learned chain
+    const int n_learned = a.max_iter < LEARNED_STEPS ? a.max_iter : LEARNED_STEPS;
     for (int i = 0; i < a.max_iter; i++) {
+        if (LEARNED) {
+#pragma unroll
+            for (int d = 0; d < 2; d++)
+                if (learned[d] && i >= n_learned) ch[0][d].active = false;
+        }
         bool any = false;
 #pragma unroll
         for (int c = 0; c < NC; c++) any = any || ch[c][0].active || ch[c][1].active;
@@ -420,8 +429,8 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
         if (LEARNED) steps += (uint32_t)ch[0][0].active + (uint32_t)ch[0][1].active;
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            chain_step(ch[c][0], i, a.max_iter, pcs, a.fwd[c]);
-            chain_step(ch[c][1], i, a.max_iter, pcs, a.bwd[c]);
+            chain_step(ch[c][0], i, (LEARNED && c == 0 && learned[0]) ? n_learned : a.max_iter, pcs, a.fwd[c]);
+            chain_step(ch[c][1], i, (LEARNED && c == 0 && learned[1]) ? n_learned : a.max_iter, pcs, a.bwd[c]);
         }
     }
     // depth, reprojection error and visibility at the found pixels (:338-341)

@@ -82,41 +82,60 @@ __device__ __forceinline__ uint16_t load_src1(const Src &s, int n, int y, int x,
 }
 
 // out[n][y][x][co] = act(bias[co] + sum_{ky,kx,ci} w[co][ky][kx][ci] * in[n][y+ky-p][x+kx-p][ci])
-template <int NT>
+// A wave computes MT groups of 16 consecutive pixels of a row x NT*16
+// outputs: each weight fragment loaded for a K step feeds MT MFMAs and each
+// input fragment NT MFMAs (register blocking; the K order per output, and so
+// the result, does not depend on MT).
+template <int NT, int MT>
 __global__ __launch_bounds__(256) void conv_kernel(Src s, const uint16_t *__restrict__ w, const float *__restrict__ bias,
                                                    int cin, int ks, int N, int H, int W, uint16_t *out_bf, float *out_f,
                                                    int relu) {
     const int lane = threadIdx.x & 63;
-    const int chunks = (W + 15) >> 4;
+    constexpr int PXW = 16 * MT;
+    const int chunks = (W + PXW - 1) / PXW;
     const long wave = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (wave >= (long)N * H * chunks) return;
     const int n = (int)(wave / ((long)H * chunks));
     const int rem = (int)(wave - (long)n * H * chunks);
-    const int y = rem / chunks, x0 = (rem - y * chunks) * 16;
+    const int y = rem / chunks, x0 = (rem - y * chunks) * PXW;
     const int pad = ks >> 1;
-    const int px = x0 + (lane & 15), kq = (lane >> 4) * 4;
+    const int kq = (lane >> 4) * 4;
     constexpr int COUT = NT * 16;
-    f32x4 acc[NT];
+    f32x4 acc[MT][NT];
 #pragma unroll
-    for (int t = 0; t < NT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int m = 0; m < MT; m++)
+#pragma unroll
+        for (int t = 0; t < NT; t++) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (int ky = 0; ky < ks; ky++) {
         const int yy = y + ky - pad;
         for (int kx = 0; kx < ks; kx++) {
-            const int xx = px + kx - pad;
-            const bool in = yy >= 0 && yy < H && xx >= 0 && xx < W && px < W;
+            bool in[MT];
+            int xx[MT];
+#pragma unroll
+            for (int m = 0; m < MT; m++) {
+                const int px = x0 + m * 16 + (lane & 15);
+                xx[m] = px + kx - pad;
+                in[m] = yy >= 0 && yy < H && xx[m] >= 0 && xx[m] < W && px < W;
+            }
+            const uint16_t *wp0 = w + (size_t)((lane & 15) * ks * ks + ky * ks + kx) * cin + kq;
             for (int c0 = 0; c0 < cin; c0 += 16) {
-                bf16x4 av = {0, 0, 0, 0};
-                if (in) av = load_src4(s, n, yy, xx, H, W, c0 + kq);
-                const uint16_t *wp = w + (size_t)((lane & 15) * ks * ks + ky * ks + kx) * cin + c0 + kq;
+                bf16x4 av[MT];
+#pragma unroll
+                for (int m = 0; m < MT; m++) {
+                    av[m] = bf16x4{0, 0, 0, 0};
+                    if (in[m]) av[m] = load_src4(s, n, yy, xx[m], H, W, c0 + kq);
+                }
 #pragma unroll
                 for (int t = 0; t < NT; t++) {
-                    const uint2 bv2 = *reinterpret_cast<const uint2 *>(wp + (size_t)t * 16 * ks * ks * cin);
+                    const uint2 bv2 = *reinterpret_cast<const uint2 *>(wp0 + c0 + (size_t)t * 16 * ks * ks * cin);
                     bf16x4 bv;
                     bv[0] = (short)(bv2.x & 0xffffu);
                     bv[1] = (short)(bv2.x >> 16);
                     bv[2] = (short)(bv2.y & 0xffffu);
                     bv[3] = (short)(bv2.y >> 16);
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[t], 0, 0, 0);
+#pragma unroll
+                    for (int m = 0; m < MT; m++)
+                        acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[m], bv, acc[m][t], 0, 0, 0);
                 }
             }
         }
@@ -126,14 +145,17 @@ __global__ __launch_bounds__(256) void conv_kernel(Src s, const uint16_t *__rest
         const int co = t * 16 + (lane & 15);
         const float bsv = bias ? bias[co] : 0.0f;
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int x = x0 + kq + i;
-            if (x >= W) continue;
-            float v = acc[t][i] + bsv;
-            if (relu) v = v > 0.0f ? v : 0.0f;
-            const size_t o = (((size_t)n * H + y) * W + x) * COUT + co;
-            if (out_f) out_f[o] = v;
-            else out_bf[o] = f2bf(v);
+        for (int m = 0; m < MT; m++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int x = x0 + m * 16 + kq + i;
+                if (x >= W) continue;
+                float v = acc[m][t][i] + bsv;
+                if (relu) v = v > 0.0f ? v : 0.0f;
+                const size_t o = (((size_t)n * H + y) * W + x) * COUT + co;
+                if (out_f) out_f[o] = v;
+                else out_bf[o] = f2bf(v);
+            }
         }
     }
 }
@@ -334,21 +356,39 @@ unsigned blocks_for(long n, K) {
     return (unsigned)((n + 255) / 256);
 }
 
+// Pixel groups per wave: wide rows get 4 (or 2) groups while the
+// accumulators (MT * NT tiles) stay within 16 tiles.
+template <int NT>
+hipError_t launch_conv_nt(const Src &s, const uint16_t *w, const float *bias, int cin, int ks, int N, int H, int W,
+                          uint16_t *out_bf, float *out_f, int relu, hipStream_t st) {
+    constexpr int MT_MAX = NT <= 4 ? 4 : (NT <= 8 ? 2 : 1);
+    const int mt = W >= 192 ? MT_MAX : (W >= 64 ? (MT_MAX < 2 ? MT_MAX : 2) : 1);
+    const long waves = (long)N * H * ((W + 16 * mt - 1) / (16 * mt));
+    const dim3 grid((unsigned)((waves + 3) / 4)), block(256);
+    if (mt == 4)
+        hipLaunchKernelGGL((conv_kernel<NT, (MT_MAX >= 4 ? 4 : 1)>), grid, block, 0, st, s, w, bias, cin, ks, N, H, W,
+                           out_bf, out_f, relu);
+    else if (mt == 2)
+        hipLaunchKernelGGL((conv_kernel<NT, (MT_MAX >= 2 ? 2 : 1)>), grid, block, 0, st, s, w, bias, cin, ks, N, H, W,
+                           out_bf, out_f, relu);
+    else
+        hipLaunchKernelGGL((conv_kernel<NT, 1>), grid, block, 0, st, s, w, bias, cin, ks, N, H, W, out_bf, out_f,
+                           relu);
+    return hipGetLastError();
+}
+
 hipError_t launch_conv(const Src &s, const uint16_t *w, const float *bias, int cin, int cout, int ks, int N, int H,
                        int W, uint16_t *out_bf, float *out_f, int relu, hipStream_t st) {
-    const long waves = (long)N * H * ((W + 15) / 16);
-    const dim3 grid((unsigned)((waves + 3) / 4)), block(256);
     switch (cout / 16) {
-        case 1: hipLaunchKernelGGL(conv_kernel<1>, grid, block, 0, st, s, w, bias, cin, ks, N, H, W, out_bf, out_f, relu); break;
-        case 2: hipLaunchKernelGGL(conv_kernel<2>, grid, block, 0, st, s, w, bias, cin, ks, N, H, W, out_bf, out_f, relu); break;
-        case 3: hipLaunchKernelGGL(conv_kernel<3>, grid, block, 0, st, s, w, bias, cin, ks, N, H, W, out_bf, out_f, relu); break;
-        case 4: hipLaunchKernelGGL(conv_kernel<4>, grid, block, 0, st, s, w, bias, cin, ks, N, H, W, out_bf, out_f, relu); break;
-        case 6: hipLaunchKernelGGL(conv_kernel<6>, grid, block, 0, st, s, w, bias, cin, ks, N, H, W, out_bf, out_f, relu); break;
-        case 8: hipLaunchKernelGGL(conv_kernel<8>, grid, block, 0, st, s, w, bias, cin, ks, N, H, W, out_bf, out_f, relu); break;
-        case 12: hipLaunchKernelGGL(conv_kernel<12>, grid, block, 0, st, s, w, bias, cin, ks, N, H, W, out_bf, out_f, relu); break;
+        case 1: return launch_conv_nt<1>(s, w, bias, cin, ks, N, H, W, out_bf, out_f, relu, st);
+        case 2: return launch_conv_nt<2>(s, w, bias, cin, ks, N, H, W, out_bf, out_f, relu, st);
+        case 3: return launch_conv_nt<3>(s, w, bias, cin, ks, N, H, W, out_bf, out_f, relu, st);
+        case 4: return launch_conv_nt<4>(s, w, bias, cin, ks, N, H, W, out_bf, out_f, relu, st);
+        case 6: return launch_conv_nt<6>(s, w, bias, cin, ks, N, H, W, out_bf, out_f, relu, st);
+        case 8: return launch_conv_nt<8>(s, w, bias, cin, ks, N, H, W, out_bf, out_f, relu, st);
+        case 12: return launch_conv_nt<12>(s, w, bias, cin, ks, N, H, W, out_bf, out_f, relu, st);
         default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 // input of layer l in the forward pass

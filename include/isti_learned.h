@@ -101,8 +101,9 @@ isti_status isti_learned_loss(isti_renderer *r, const float *out, const uint32_t
  * floor((x + 0.5 - m.x * W/2) + o.x * 2^k), likewise in y, clamped to the
  * view: one analytic search step from the pixel itself (m = the direction's
  * cam_ff at the pixel, in NDC) corrected by the network's residual o
- * (channels 0/1, in cells of 2^k pixels). A learned result is kept when its
- * search ends with an error below EPS_1 on a surface not behind the window
+ * (channels 0/1, in cells of 2^k pixels). A learned start's search runs at
+ * most 4 steps (fewer if the frame's cap is lower). A learned result is kept
+ * when that search ends with an error below EPS_1 on a surface not behind the window
  * start's candidate (depth_to_z difference < EPS_2, the splat's depth test);
  * a rejected learned start, and every low-confidence pixel, uses the
  * reference's window start instead. conf_threshold 1 disables the learned starts (the

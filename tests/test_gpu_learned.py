@@ -14,6 +14,7 @@ pytestmark = pytest.mark.gpu
 
 EPS_1 = np.float32(0.002)  # EPS_1_FULL_RES, src/renderer.cu:16
 EPS_2 = np.float32(0.001)  # EPS_2_FULL_RES, src/renderer.cu:18
+LEARNED_STEPS = 4  # search steps granted to a learned start
 NONE = np.uint32(0xFFFFFFFF)
 F1, HALF, TWO = np.float32(1), np.float32(0.5), np.float32(2)
 
@@ -46,7 +47,7 @@ def _expected_capture(O, sc, zs, starts_learned, learned_mask, iters, alpha):
         qp, ep = O.search_field(cam, ws, iters)
         plain = np.where(ep < EPS_1, qp, NONE)
         ls = np.where(learned_mask[d], starts_learned[d], NONE)
-        ql, el = O.search_field(cam, ls, iters)
+        ql, el = O.search_field(cam, ls, min(iters, LEARNED_STEPS))
         qx, qy = (ql & 0xFFFF).astype(np.int64), (ql >> 16).astype(np.int64)
         ok = ql != NONE
         qx, qy = np.where(ok, qx, 0), np.where(ok, qy, 0)
