@@ -300,11 +300,12 @@ def main():
         ri_mid = R.RenderInfo((W, H), infos[len(infos) // 2].alpha, np.ctypeslib.as_array(infos[len(infos) // 2].viewproj),
                               np.ctypeslib.as_array(infos[len(infos) // 2].inv_viewproj),
                               R.AdditionalRenderInfo(fpa, sfpa, False, R.RenderMode(args.mode)), args.iters)
-        best, tms = rend.autotune(out, ri_mid, reps=20)
-        tuned = {"tile": list(N.TILE_SHAPES[best]),
-                 "resolve_ms": {f"{a}x{b}": round(t, 5) for (a, b), t in zip(N.TILE_SHAPES, tms)}}
-        print(f"[bench] autotuned resolve tile {N.TILE_SHAPES[best]}: {tuned['resolve_ms']}", file=sys.stderr,
-              flush=True)
+        at = rend.autotune(out, ri_mid, reps=20)
+        tuned = {"tile": list(N.TILE_SHAPES[at["tile"]]),
+                 "resolve_ms": {f"{a}x{b}": round(t, 5) for (a, b), t in zip(N.TILE_SHAPES, at["resolve_ms"])},
+                 "splat": "lds-binned" if at["binned"] else "direct",
+                 "clear_splat_ms": {"direct": round(at["splat_ms"][0], 5), "lds-binned": round(at["splat_ms"][1], 5)}}
+        print(f"[bench] autotuned: {tuned}", file=sys.stderr, flush=True)
 
     n_frames = args.steps * len(infos)
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n_frames)]

@@ -42,8 +42,10 @@ EXPORTED_SYMBOLS = (
     "isti_batch_last_error",
     "isti_autotune",
     "isti_set_tile_shape",
+    "isti_set_splat_binning",
 )
 ISTI_N_TILE_SHAPES = 4
+ISTI_AUTOTUNE_TIMES = ISTI_N_TILE_SHAPES + 2
 TILE_SHAPES = ((16, 16), (32, 8), (64, 4), (8, 32))
 
 
@@ -159,7 +161,9 @@ def lib() -> ctypes.CDLL:
     L.isti_batch_last_error.argtypes = [c_void_p]
     L.isti_batch_last_error.restype = c_char_p
     L.isti_autotune.argtypes = [c_void_p, POINTER(RenderInfoC), c_void_p, c_size_t, c_int32, POINTER(c_float),
-                                POINTER(c_int32), c_void_p]
+                                POINTER(c_int32), POINTER(c_int32), c_void_p]
+    L.isti_set_splat_binning.argtypes = [c_void_p, c_int32]
+    L.isti_set_splat_binning.restype = c_int32
     L.isti_autotune.restype = c_int32
     L.isti_set_tile_shape.argtypes = [c_void_p, c_int32]
     L.isti_set_tile_shape.restype = c_int32

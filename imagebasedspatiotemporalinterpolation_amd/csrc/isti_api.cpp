@@ -33,10 +33,12 @@ namespace {
 
 thread_local std::string g_create_error;
 
-// Tile shapes chosen by isti_autotune, per (width, height, n_aux, window).
+// Tile shapes and splat variants chosen by isti_autotune, per (width,
+// height, n_aux, window).
 using TuneKey = std::tuple<uint32_t, uint32_t, int, int>;
 std::mutex g_tuned_mutex;
 std::map<TuneKey, int> g_tuned;
+std::map<TuneKey, int> g_tuned_splat;
 
 struct Scratch {
     uint32_t w = 0, h = 0;
@@ -76,6 +78,7 @@ struct isti_renderer {
     isti_render_info prepared{};
     bool prepared_valid = false;
     int tile_shape = -1;  // forced resolve tile shape, or -1 (tuned / default)
+    int splat_mode = -1;  // forced splat variant (0 direct, 1 LDS-binned), or -1 (tuned / direct)
     std::string err;
 };
 
@@ -372,9 +375,16 @@ int tile_shape_for(const isti_renderer *r, const FramePlan &P) {
     return it == g_tuned.end() ? 0 : it->second;
 }
 
+bool splat_binned_for(const isti_renderer *r, const FramePlan &P) {
+    if (r->splat_mode >= 0) return r->splat_mode == 1;
+    std::lock_guard<std::mutex> lock(g_tuned_mutex);
+    const auto it = g_tuned_splat.find(TuneKey{P.ra.w, P.ra.h, P.ra.n_aux, P.window});
+    return it != g_tuned_splat.end() && it->second == 1;
+}
+
 // M1 clear and K1 reprojection + splat of every view of a planned frame
 // (src/renderer.cu:631-649), with the stage-timing marks 0, 1 and 2.
-isti_status splat_stage(isti_renderer *r, const FramePlan &P, hipStream_t s) {
+isti_status splat_stage(isti_renderer *r, const FramePlan &P, hipStream_t s, bool binned) {
     ClearArgs ca;
     std::memset(&ca, 0, sizeof(ca));
     SplatArgs sa;
@@ -383,7 +393,7 @@ isti_status splat_stage(isti_renderer *r, const FramePlan &P, hipStream_t s) {
     sa.vp_main[0] = P.vp_main[0];
     sa.vp_main[1] = P.vp_main[1];
     sa.init_linear = P.init_linear;
-    uint32_t clear_blocks = 0, splat_blocks = 0;
+    uint32_t clear_blocks = 0, n_splat_blocks = 0;
     for (int j = 0; j < P.nj; j++) {
         SplatJob &J = sa.job[j];
         J = P.jobs[j];
@@ -391,18 +401,18 @@ isti_status splat_stage(isti_renderer *r, const FramePlan &P, hipStream_t s) {
         ca.n[j] = J.n_px;
         ca.block_begin[j] = clear_blocks;
         clear_blocks += (J.n_px + 511u) / 512u;
-        J.block_begin = splat_blocks;
-        splat_blocks += (J.n_px + SPLAT_BLOCK - 1) / SPLAT_BLOCK;
+        J.block_begin = n_splat_blocks;
+        n_splat_blocks += splat_blocks(J.w, J.h, binned);
     }
     ca.n_bufs = P.nj;
     ca.n_blocks = clear_blocks;
     sa.n_jobs = P.nj;
-    sa.n_blocks = splat_blocks;
+    sa.n_blocks = n_splat_blocks;
     hipError_t e;
     if ((e = mark(r, 0, s)) != hipSuccess) return dev_fail(r, e, "event");
     if ((e = launch_clear(ca, s)) != hipSuccess) return dev_fail(r, e, "clear launch");
     if ((e = mark(r, 1, s)) != hipSuccess) return dev_fail(r, e, "event");
-    if ((e = launch_splat(sa, s)) != hipSuccess) return dev_fail(r, e, "splat launch");
+    if ((e = launch_splat(sa, binned, s)) != hipSuccess) return dev_fail(r, e, "splat launch");
     if ((e = mark(r, 2, s)) != hipSuccess) return dev_fail(r, e, "event");
     return ISTI_OK;
 }
@@ -679,7 +689,7 @@ isti_status isti_learned_features(isti_renderer *r, const isti_render_info *info
     st = plan_frame(r, info, nullptr, (size_t)info->width * 4, P);
     if (st != ISTI_OK) return st;
     r->prepared_valid = false;
-    st = splat_stage(r, P, s);
+    st = splat_stage(r, P, s, splat_binned_for(r, P));
     if (st != ISTI_OK) return st;
     for (int sl = 0; sl < 2; sl++) fa.v[sl] = sl == 0 ? P.ra.fwd[0] : P.ra.bwd[0];
     fa.x = x;
@@ -835,7 +845,7 @@ isti_status isti_render(isti_renderer *r, const isti_render_info *info, uint8_t 
     const int shape = tile_shape_for(r, P);
     set_tiles(P.ra, shape);
     r->prepared_valid = false;
-    st = splat_stage(r, P, s);
+    st = splat_stage(r, P, s, splat_binned_for(r, P));
     if (st != ISTI_OK) return st;
     e = launch_resolve(P.ra, P.window, P.always_valid, shape, s);
     if (e != hipSuccess) return dev_fail(r, e, "resolve launch");
@@ -862,7 +872,7 @@ isti_status isti_set_tile_shape(isti_renderer *r, int32_t shape) {
 }
 
 isti_status isti_autotune(isti_renderer *r, const isti_render_info *info, uint8_t *out, size_t out_pitch,
-                          int32_t reps, float *times_ms, int32_t *best, void *stream) {
+                          int32_t reps, float *times_ms, int32_t *best, int32_t *best_splat, void *stream) {
     if (!r) return ISTI_ERR_INVALID_ARG;
     if (!info || !out || reps < 1 || reps > 1000)
         return fail(r, ISTI_ERR_INVALID_ARG, "info and out must be non-null and reps in 1..1000");
@@ -878,12 +888,28 @@ isti_status isti_autotune(isti_renderer *r, const isti_render_info *info, uint8_
     r->prepared_valid = false;
     const bool timing = r->timing;
     r->timing = false;  // the tuning launches stay out of the stage statistics
-    st = splat_stage(r, P, s);
-    r->timing = timing;
-    if (st != ISTI_OK) return st;
     hipEvent_t ev[2] = {nullptr, nullptr};
     e = hipEventCreate(&ev[0]);
     if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+    // splat stage (clear + reprojection/splat) with each splat variant
+    float splat_ms[2] = {};
+    for (int v = 0; v < 2 && e == hipSuccess; v++) {
+        if ((st = splat_stage(r, P, s, v == 1)) != ISTI_OK) break;  // warm-up
+        if ((e = hipEventRecord(ev[0], s)) != hipSuccess) break;
+        for (int i = 0; i < reps && st == ISTI_OK; i++) st = splat_stage(r, P, s, v == 1);
+        if (st != ISTI_OK) break;
+        if ((e = hipEventRecord(ev[1], s)) == hipSuccess) e = hipEventSynchronize(ev[1]);
+        if (e == hipSuccess) e = hipEventElapsedTime(&splat_ms[v], ev[0], ev[1]);
+        splat_ms[v] /= (float)reps;
+    }
+    const int pick_splat = splat_ms[1] < splat_ms[0] ? 1 : 0;
+    if (st == ISTI_OK && e == hipSuccess) st = splat_stage(r, P, s, pick_splat == 1);  // the frame's buffers
+    r->timing = timing;
+    if (st != ISTI_OK) {
+        for (hipEvent_t x : ev)
+            if (x) (void)hipEventDestroy(x);
+        return st;
+    }
     float ms[N_TILE_SHAPES] = {};
     int pick = 0;
     for (int k = 0; k < N_TILE_SHAPES && e == hipSuccess; k++) {
@@ -903,9 +929,22 @@ isti_status isti_autotune(isti_renderer *r, const isti_render_info *info, uint8_
     {
         std::lock_guard<std::mutex> lock(g_tuned_mutex);
         g_tuned[TuneKey{P.ra.w, P.ra.h, P.ra.n_aux, P.window}] = pick;
+        g_tuned_splat[TuneKey{P.ra.w, P.ra.h, P.ra.n_aux, P.window}] = pick_splat;
     }
-    if (times_ms) std::memcpy(times_ms, ms, sizeof(ms));
+    if (times_ms) {
+        std::memcpy(times_ms, ms, sizeof(ms));
+        times_ms[N_TILE_SHAPES] = splat_ms[0];
+        times_ms[N_TILE_SHAPES + 1] = splat_ms[1];
+    }
     if (best) *best = pick;
+    if (best_splat) *best_splat = pick_splat;
+    return ISTI_OK;
+}
+
+isti_status isti_set_splat_binning(isti_renderer *r, int32_t mode) {
+    if (!r) return ISTI_ERR_INVALID_ARG;
+    if (mode < -1 || mode > 1) return fail(r, ISTI_ERR_INVALID_ARG, "splat mode must be -1, 0 or 1");
+    r->splat_mode = mode;
     return ISTI_OK;
 }
 

@@ -48,12 +48,12 @@ __global__ __launch_bounds__(256) void clear_splat_kernel(const ClearArgs a) {
 // float4 flow stores are fully coalesced; the u64 atomicMin is the only
 // scattered access.
 // ---------------------------------------------------------------------------
+// Reprojection of source pixel (x, y) (p = y w + x): writes full_ff and
+// cam_ff and returns whether it splats, with its target pixel and key.
 template <typename J_t, typename M_t>
-__device__ __forceinline__ void splat_pixel(const J_t &J, const M_t &vp_cam, const M_t &vp_main, bool init_linear,
-                                            uint32_t p) {
-    if (p >= J.n_px) return;
-    const uint32_t y = p / J.w;
-    const uint32_t x = p - y * J.w;
+__device__ __forceinline__ bool project_pixel(const J_t &J, const M_t &vp_cam, const M_t &vp_main, bool init_linear,
+                                              uint32_t x, uint32_t y, uint32_t p, uint32_t &tx, uint32_t &ty,
+                                              unsigned long long &key) {
 
     const float z_from = J.z[p];
     float4 om = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -78,20 +78,35 @@ __device__ __forceinline__ void splat_pixel(const J_t &J, const M_t &vp_cam, con
     J.full_ff[p] = make_float4(fx, fy, fz, vis);
 
     float depth = z_to_depth(cs_from.z);
-    uint32_t tx = x, ty = y;
+    tx = x;
+    ty = y;
     if (!init_linear) {
         const float4 cs_cam = project(vp_cam, ws_scaled);
         J.cam_ff[p] = make_float4(cs_cam.x - cs_from.x, cs_cam.y - cs_from.y, cs_cam.z - cs_from.z, vis);
         const float depth_cam = z_to_depth(cs_cam.z);
         const int2 pc = cs2pixel(cs_cam.x, cs_cam.y, J.w, J.h);
-        if (!in_range(pc, J.w, J.h) || depth_cam < 0.0f) return;
+        if (!in_range(pc, J.w, J.h) || depth_cam < 0.0f) return false;
         depth = depth_cam;
         tx = (uint32_t)pc.x;
         ty = (uint32_t)pc.y;
     } else {
         J.cam_ff[p] = make_float4(fx * wt, fy * wt, fz * wt, vis);
     }
-    atomicMin(&J.splat[ty * J.w + tx], encode_key(depth, x, y));
+    key = encode_key(depth, x, y);
+    return true;
+}
+
+// Direct splat: one 64-bit atomicMin into HBM per splatting source pixel.
+template <typename J_t, typename M_t>
+__device__ __forceinline__ void splat_pixel(const J_t &J, const M_t &vp_cam, const M_t &vp_main, bool init_linear,
+                                            uint32_t p) {
+    if (p >= J.n_px) return;
+    const uint32_t y = p / J.w;
+    const uint32_t x = p - y * J.w;
+    uint32_t tx, ty;
+    unsigned long long key;
+    if (project_pixel(J, vp_cam, vp_main, init_linear, x, y, p, tx, ty, key))
+        atomicMin(&J.splat[ty * J.w + tx], key);
 }
 
 __global__ __launch_bounds__(256) void reproject_splat_kernel(const SplatArgs a) {
@@ -101,6 +116,47 @@ __global__ __launch_bounds__(256) void reproject_splat_kernel(const SplatArgs a)
         if (b >= a.job[i].block_begin) j = i;
     const SplatJob &J = a.job[j];
     splat_pixel(J, a.vp_cam, a.vp_main[J.main_slot], a.init_linear != 0, (b - J.block_begin) * SPLAT_BLOCK + threadIdx.x);
+}
+
+// LDS-binned splat: a block reprojects a 32x8 source tile and first
+// resolves the atomicMin of every target inside a bin of the tile's
+// footprint +-16 pixels in LDS (64-bit ds_min), sending targets outside the
+// bin straight to HBM; then each non-empty bin cell costs one global
+// atomicMin. The minimum is order-independent, so the buffer is the same
+// as the direct splat's. It pays where many sources land on few cells.
+constexpr int BIN_TW = 32, BIN_TH = 8, BIN_M = 16;
+constexpr int BIN_W = BIN_TW + 2 * BIN_M, BIN_H = BIN_TH + 2 * BIN_M;
+
+__global__ __launch_bounds__(256) void reproject_splat_binned_kernel(const SplatArgs a) {
+    __shared__ unsigned long long bin[BIN_H * BIN_W];
+    const uint32_t b = xcd_remap(blockIdx.x, a.n_blocks);
+    int j = 0;
+    for (int i = 1; i < a.n_jobs; i++)
+        if (b >= a.job[i].block_begin) j = i;
+    const SplatJob &J = a.job[j];
+    const uint32_t tiles_x = (J.w + BIN_TW - 1) / BIN_TW;
+    const uint32_t t = b - J.block_begin, tyi = t / tiles_x;
+    const int x0 = (int)((t - tyi * tiles_x) * BIN_TW), y0 = (int)(tyi * BIN_TH);
+    for (int i = (int)threadIdx.x; i < BIN_H * BIN_W; i += 256) bin[i] = SPLAT_EMPTY;
+    __syncthreads();
+    const uint32_t x = (uint32_t)x0 + (threadIdx.x % BIN_TW), y = (uint32_t)y0 + (threadIdx.x / BIN_TW);
+    if (x < J.w && y < J.h) {
+        uint32_t tx, ty;
+        unsigned long long key;
+        if (project_pixel(J, a.vp_cam, a.vp_main[J.main_slot], a.init_linear != 0, x, y, y * J.w + x, tx, ty, key)) {
+            const int bx = (int)tx - (x0 - BIN_M), by = (int)ty - (y0 - BIN_M);
+            if (bx >= 0 && bx < BIN_W && by >= 0 && by < BIN_H) atomicMin(&bin[by * BIN_W + bx], key);
+            else atomicMin(&J.splat[ty * J.w + tx], key);
+        }
+    }
+    __syncthreads();
+    for (int i = (int)threadIdx.x; i < BIN_H * BIN_W; i += 256) {
+        const unsigned long long k = bin[i];
+        if (k != SPLAT_EMPTY) {  // only in-view targets were binned
+            const int gx = x0 - BIN_M + i % BIN_W, gy = y0 - BIN_M + i / BIN_W;
+            atomicMin(&J.splat[(uint32_t)gy * J.w + (uint32_t)gx], k);
+        }
+    }
 }
 
 // Learned search starts: the same resolve, with the main camera's searches
@@ -411,10 +467,17 @@ hipError_t launch_clear(const ClearArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_splat(const SplatArgs &a, hipStream_t s) {
+hipError_t launch_splat(const SplatArgs &a, bool binned, hipStream_t s) {
     if (a.n_blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(reproject_splat_kernel, dim3(a.n_blocks), dim3(SPLAT_BLOCK), 0, s, a);
+    if (binned)
+        hipLaunchKernelGGL(reproject_splat_binned_kernel, dim3(a.n_blocks), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(reproject_splat_kernel, dim3(a.n_blocks), dim3(SPLAT_BLOCK), 0, s, a);
     return hipGetLastError();
+}
+
+uint32_t splat_blocks(uint32_t w, uint32_t h, bool binned) {
+    return binned ? ((w + BIN_TW - 1) / BIN_TW) * ((h + BIN_TH - 1) / BIN_TH) : (w * h + SPLAT_BLOCK - 1) / SPLAT_BLOCK;
 }
 
 hipError_t launch_features(const FeatureArgs &a, hipStream_t s) {

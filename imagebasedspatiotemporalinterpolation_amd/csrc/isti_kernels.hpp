@@ -159,7 +159,10 @@ hipError_t launch_learned_loss(const float *out, const uint32_t *q, const float4
 hipError_t launch_resolve_learned(const ResolveArgs &a, hipStream_t s);
 
 hipError_t launch_clear(const ClearArgs &a, hipStream_t s);
-hipError_t launch_splat(const SplatArgs &a, hipStream_t s);
+// binned: the LDS-binned variant; its blocks are 32x8 source tiles, so a
+// job's block_begin prefix must come from splat_blocks(..., true)
+hipError_t launch_splat(const SplatArgs &a, bool binned, hipStream_t s);
+uint32_t splat_blocks(uint32_t w, uint32_t h, bool binned);
 hipError_t launch_resolve(const ResolveArgs &a, int window, bool always_valid, int shape, hipStream_t s);
 hipError_t launch_mean_z(const float *z, uint32_t w, uint32_t h, double *row_sums, float *mean, hipStream_t s);
 hipError_t launch_timewarp(const TimewarpArgs &a, hipStream_t s);

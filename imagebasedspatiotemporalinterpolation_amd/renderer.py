@@ -191,20 +191,28 @@ class Renderer:
         N.check(N.lib().isti_render(self._h, ctypes.byref(ci), render_buffer.data_ptr(), w * 4,
                                     ctypes.c_void_p(s.cuda_stream)), self._h)
 
-    def autotune(self, render_buffer, render_info: RenderInfo, reps: int = 20, stream=None) -> tuple:
-        """Times every resolve tile shape on this frame and keeps the fastest
-        for renders of its size, aux count and window; returns (best shape,
-        per-shape ms). Synchronises the stream."""
+    def autotune(self, render_buffer, render_info: RenderInfo, reps: int = 20, stream=None) -> dict:
+        """Times the splat variants and every resolve tile shape on this frame
+        and keeps the fastest for renders of its size, aux count and window.
+        Returns {"tile": shape index, "resolve_ms": [per shape], "binned": 0/1,
+        "splat_ms": [direct, binned]}. Synchronises the stream."""
         import torch
 
         h, w = render_buffer.shape[:2]
         s = stream if stream is not None else torch.cuda.current_stream()
         ci = render_info.to_c()
-        ms = (ctypes.c_float * N.ISTI_N_TILE_SHAPES)()
-        best = ctypes.c_int32(-1)
+        ms = (ctypes.c_float * N.ISTI_AUTOTUNE_TIMES)()
+        best, best_splat = ctypes.c_int32(-1), ctypes.c_int32(-1)
         N.check(N.lib().isti_autotune(self._h, ctypes.byref(ci), render_buffer.data_ptr(), w * 4, int(reps), ms,
-                                      ctypes.byref(best), ctypes.c_void_p(s.cuda_stream)), self._h)
-        return best.value, [float(v) for v in ms]
+                                      ctypes.byref(best), ctypes.byref(best_splat), ctypes.c_void_p(s.cuda_stream)),
+                self._h)
+        t = [float(v) for v in ms]
+        return {"tile": best.value, "resolve_ms": t[:N.ISTI_N_TILE_SHAPES], "binned": best_splat.value,
+                "splat_ms": t[N.ISTI_N_TILE_SHAPES:]}
+
+    def set_splat_binning(self, mode: int) -> None:
+        """Force the splat variant: 0 direct, 1 LDS-binned, -1 tuned."""
+        N.check(N.lib().isti_set_splat_binning(self._h, int(mode)), self._h)
 
     def set_tile_shape(self, shape: int) -> None:
         """Force a resolve tile shape (index into _native.TILE_SHAPES), or -1 for the tuned one."""

@@ -170,14 +170,29 @@ isti_status isti_render(isti_renderer *r, const isti_render_info *info,
  * in the reference (16x16 blocks, src/renderer.cu:676-693). */
 #define ISTI_N_TILE_SHAPES 4
 
-/* Autotuner: renders frame `info` once, then times the resolve pass of every
- * tile shape (`reps` launches each, HIP events) and keeps the fastest for
- * every later isti_render of the same render size, aux count and search
- * window, in this process. times_ms (optional, ISTI_N_TILE_SHAPES floats)
- * receives the average resolve time per shape, *best (optional) the shape
- * chosen. `out_rgba8` receives the frame. Synchronises `stream`. */
+/* Splat variants: 0 = direct (one 64-bit atomicMin into HBM per splatting
+ * source pixel), 1 = LDS-binned (32x8 source tiles resolve their atomicMin
+ * in an LDS bin of the tile's footprint +-16 pixels first, then one HBM
+ * atomicMin per non-empty bin cell; targets outside the bin go to HBM).
+ * Both leave identical splat buffers. No counterpart in the reference
+ * (direct atomicMin, src/renderer.cu:144-145). */
+#define ISTI_AUTOTUNE_TIMES (ISTI_N_TILE_SHAPES + 2)
+
+/* Autotuner: times frame `info`'s splat stage (clear + reprojection/splat)
+ * with each splat variant and its resolve pass with every tile shape
+ * (`reps` launches each, HIP events) and keeps the fastest of each for every
+ * later isti_render of the same render size, aux count and search window,
+ * in this process. times_ms (optional, ISTI_AUTOTUNE_TIMES floats) receives
+ * the average resolve time per tile shape, then the splat-stage time of the
+ * direct and the binned variant; *best / *best_splat (optional) the choices.
+ * `out_rgba8` receives the frame. Synchronises `stream`. */
 isti_status isti_autotune(isti_renderer *r, const isti_render_info *info, uint8_t *out_rgba8,
-                          size_t out_pitch_bytes, int32_t reps, float *times_ms, int32_t *best, void *stream);
+                          size_t out_pitch_bytes, int32_t reps, float *times_ms, int32_t *best,
+                          int32_t *best_splat, void *stream);
+
+/* Forces r's splat variant (0 direct, 1 LDS-binned); -1 returns to the
+ * tuned one, or direct where nothing was tuned. */
+isti_status isti_set_splat_binning(isti_renderer *r, int32_t mode);
 
 /* Forces the resolve tile shape of r's renders (0..ISTI_N_TILE_SHAPES-1);
  * -1 returns to the tuned shape, or 16x16 where nothing was tuned. */

@@ -58,11 +58,14 @@ def test_autotune_times_every_shape_and_later_renders_use_it():
     base = torch.zeros((h, w, 4), dtype=torch.uint8, device="cuda")
     r.render(base, info)
     out = torch.zeros_like(base)
-    best, ms = r.autotune(out, info, reps=5)
+    at = r.autotune(out, info, reps=5)
     torch.cuda.synchronize()
-    print("tile shape times (ms):", dict(zip(N.TILE_SHAPES, ms)), "best", N.TILE_SHAPES[best])
+    best, ms = at["tile"], at["resolve_ms"]
+    print("tile shape times (ms):", dict(zip(N.TILE_SHAPES, ms)), "best", N.TILE_SHAPES[best],
+          "splat (direct, binned):", at["splat_ms"])
     assert 0 <= best < N.ISTI_N_TILE_SHAPES
     assert all(t > 0 for t in ms) and ms[best] == min(ms)
+    assert at["binned"] in (0, 1) and at["splat_ms"][at["binned"]] == min(at["splat_ms"])
     assert torch.equal(out, base)  # the autotuner's frame is the frame
     # a fresh renderer of the same size, aux count and window uses the tuned shape
     r2, fd2, info2, *_ = _setup(cfg, 3, 0, 3)
@@ -74,3 +77,29 @@ def test_autotune_times_every_shape_and_later_renders_use_it():
         r2.set_tile_shape(N.ISTI_N_TILE_SHAPES)
     r.close()
     r2.close()
+
+
+# C1 at its own size, the 2-aux shadow scene at /4, and the fast-camera stereo
+# eye at /4 (large motions: many targets leave the +-16-pixel bins)
+@pytest.mark.parametrize("wl,div,b,mode,views", [(1, 1, 2, 0, 1), (2, 4, 3, 0, 3), (3, 4, 2, 0, 1), (2, 4, 5, 1, 3)])
+def test_lds_binned_splat_matches_the_oracle(oracle_lib, wl, div, b, mode, views):
+    """The LDS-binned splat leaves the oracle's splat buffers bit for bit
+    (and so the same frame) in every mode, including mode 1's linear splat."""
+    cfg = F.scaled(wl, 8, div)
+    r, fd, info, fwd, bwd, vp, ivp, (w, h, fpa, sfpa) = _setup(cfg, b, mode, views)
+    _, sref = F.oracle_frame(oracle_lib, fwd, bwd, vp, ivp, b / fpa, fpa, sfpa, mode, 8, False)
+    outs = []
+    for binned in (0, 1):
+        print(f"case: {w}x{h} aux={views - 1} mode={mode} binned={binned}", flush=True)
+        r.set_splat_binning(binned)
+        out = torch.zeros((h, w, 4), dtype=torch.uint8, device="cuda")
+        r.render(out, info)
+        torch.cuda.synchronize()
+        for key, sc in sref.items():
+            got = r.scratch(*key)
+            assert np.array_equal(got["splat"], sc["splat"]), (binned, key)
+        outs.append(out.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
+    with pytest.raises(N.IstiError):
+        r.set_splat_binning(2)
+    r.close()
