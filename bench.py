@@ -367,6 +367,15 @@ def main():
         "p50_frame_ms": round(float(np.percentile(frame_ms, 50)), 4),
         "p99_frame_ms": round(float(np.percentile(frame_ms, 99)), 4),
         "stage_avg_ms": {k2: round(v2, 5) for k2, v2 in stage_avg_ms.items()},
+        # K1: at most one 64-bit atomicMin per source pixel (every in-view one)
+        "splat": {
+            "variant": tuned["splat"] if tuned else "direct",
+            "source_px_per_launch": src_px,
+            "atomic_min_per_s_upper": round(src_px / (stage_avg_ms["splat"] * 1e-3), 1),
+            "algorithmic_gbps": round(alg["splat"] / (stage_avg_ms["splat"] * 1e-3) / 1e9, 1),
+            "traffic_gbps": (round(json.loads(tf.read_text())["splat"] / (stage_avg_ms["splat"] * 1e-3) / 1e9, 1)
+                             if tf.exists() else None),
+        },
         "roofline": {
             "kernel": dom,
             "bound": "hbm",
