@@ -144,14 +144,17 @@ def stereo_bench(n_frames: int, seed: int) -> dict:
     for i in range(10):
         frame(i)
     torch.cuda.synchronize()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n_frames)]
     t0 = time.perf_counter()
     for i in range(n_frames):
+        frame(i)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n_frames)]
+    for i in range(n_frames):  # latency pass
         ev[i][0].record(stream)
         frame(i)
         ev[i][1].record(stream)
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
     ms = np.array([a.elapsed_time(b) for a, b in ev])
     batch.close()
     return {
@@ -196,15 +199,18 @@ def learned_bench(n_frames: int, seed: int, iters: int) -> dict:
         for i in range(10):
             fn(infos[i % len(infos)])
         torch.cuda.synchronize()
-        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n_frames)]
         gc.disable()  # no collector pauses between the enqueued frames
         t0 = time.perf_counter()
         for i in range(n_frames):
+            fn(infos[i % len(infos)])
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n_frames)]
+        for i in range(n_frames):  # latency pass
             ev[i][0].record(stream)
             fn(infos[i % len(infos)])
             ev[i][1].record(stream)
         torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
         gc.enable()
         ms = np.array([a.elapsed_time(b) for a, b in ev])
         return {"mpix_per_s": round(n_frames * W * H / wall / 1e6, 1),
@@ -308,11 +314,19 @@ def main():
         print(f"[bench] autotuned: {tuned}", file=sys.stderr, flush=True)
 
     n_frames = args.steps * len(infos)
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n_frames)]
-    rend.set_stage_timing(True)
+    # 1. throughput: the K steps alone, between two synchronisations
     torch.cuda.synchronize()
     gc.disable()
     t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for ci in infos:
+            frame(ci)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gc.enable()
+    # 2. per-frame latency: HIP events around every frame (same K steps)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n_frames)]
+    gc.disable()
     k = 0
     for _ in range(args.steps):
         for ci in infos:
@@ -321,11 +335,16 @@ def main():
             ev[k][1].record(stream)
             k += 1
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
     gc.enable()
+    frame_ms = np.array([a.elapsed_time(b) for a, b in ev])
+    # 3. per-stage times from the renderer's own events on its stream
+    rend.set_stage_timing(True)
+    for _ in range(max(1, min(args.steps, 50))):
+        for ci in infos:
+            frame(ci)
+    torch.cuda.synchronize()
     totals, count = rend.stage_totals()
     rend.set_stage_timing(False)
-    frame_ms = np.array([a.elapsed_time(b) for a, b in ev])
 
     src_px = sum(int(v["z"].size) for v in fwd) + sum(int(v["z"].size) for v in bwd)
     out_px = W * H
