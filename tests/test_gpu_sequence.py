@@ -1,0 +1,119 @@
+"""Successive frames on one renderer: the scratch (splat buffers, flows)
+carried from frame to frame must leave every frame with the oracle's splat
+buffers and pixels, also when the path changes between frames (plain
+render, autotune, batch, learned-start resolve, window-3 mode) and when the
+keyframe size changes (the scratch is reallocated)."""
+import numpy as np
+import pytest
+import torch
+
+from imagebasedspatiotemporalinterpolation_amd import learned as Ln
+from imagebasedspatiotemporalinterpolation_amd import renderer as R
+from tests import _frames as F
+
+pytestmark = pytest.mark.gpu
+
+
+def _frame_data(fwd, bwd):
+    kf = [R.KeyframeView.from_numpy(v) for v in fwd]
+    kb = [R.KeyframeView.from_numpy(v) for v in bwd]
+    return R.FrameData(kf[0], kb[0], [R.AuxCamData(kf[i], kb[i], F.AUX_PRIORITY) for i in range(1, len(fwd))])
+
+
+class _Seq:
+    def __init__(self, O, cfg, r):
+        self.O, self.cfg, self.r = O, cfg, r
+        self.fwd, self.bwd = F.pair(cfg, 0)
+        self.fd = _frame_data(self.fwd, self.bwd)
+        r.set_frame(self.fd)
+        self.fpa = cfg.keyframe_spacing
+        self.sfpa = cfg.shadow_frames if cfg.shadows else 0
+
+    def info(self, b, mode=0, iters=8):
+        vp, ivp, w, h = F.target(self.cfg, 0, b)
+        ri = R.RenderInfo((w, h), b / self.fpa, vp, ivp,
+                          R.AdditionalRenderInfo(self.fpa, self.sfpa, False, R.RenderMode(mode)), iters)
+        return ri, vp, ivp, w, h
+
+    def render(self, b, mode=0):
+        ri, _, _, w, h = self.info(b, mode)
+        out = torch.zeros((h, w, 4), dtype=torch.uint8, device="cuda")
+        self.r.render(out, ri)
+        return out
+
+    def check(self, out, b, mode=0, iters=8, prev=None, what=""):
+        """Compares the frame just rendered (and the renderer's splat
+        buffers) with the oracle's frame."""
+        _, vp, ivp, w, h = self.info(b, mode, iters)
+        torch.cuda.synchronize()
+        ref, sref = F.oracle_frame(self.O, self.fwd, self.bwd, vp, ivp, b / self.fpa, self.fpa, self.sfpa, mode,
+                                   iters, False, prev)
+        for key, sc in sref.items():
+            got = self.r.scratch(*key)["splat"]
+            assert np.array_equal(got, sc["splat"]), (what, key, int((got != sc["splat"]).sum()))
+        F.assert_rgb_parity(out.cpu().numpy(), ref)
+
+
+def _prev(w, h, seed):
+    return np.random.default_rng(seed).integers(0, 256, (h, w, 4), dtype=np.uint8)
+
+
+def test_sequence_of_plain_renders(oracle_lib):
+    cfg = F.scaled(2, 7, 4)
+    r = R.Renderer(n_aux=2)
+    S = _Seq(oracle_lib, cfg, r)
+    for i, (b, mode) in enumerate([(1, 0), (2, 0), (3, 0), (4, 1), (5, 3), (2, 0)]):
+        ri, vp, ivp, w, h = S.info(b, mode)
+        prev = _prev(w, h, i)
+        print(f"case: frame {i} b={b} mode={mode} {w}x{h}", flush=True)
+        out = torch.from_numpy(prev.copy()).cuda()
+        r.render(out, ri)
+        S.check(out, b, mode, prev=prev, what=f"frame {i}")
+    r.close()
+
+
+def test_sequence_across_paths_and_sizes(oracle_lib):
+    cfg = F.scaled(2, 8, 4)
+    r = R.Renderer(n_aux=2)
+    S = _Seq(oracle_lib, cfg, r)
+    batch = R.Batch(max_requests=2)
+    print("case: plain, plain", flush=True)
+    S.check(S.render(1), 1, what="plain 1")
+    S.check(S.render(2), 2, what="plain 2")
+
+    print("case: autotune", flush=True)
+    ri, _, _, w, h = S.info(3)
+    out = torch.zeros((h, w, 4), dtype=torch.uint8, device="cuda")
+    r.autotune(out, ri, reps=2)
+    S.check(out, 3, what="autotune")
+    S.check(S.render(4), 4, what="plain after autotune")
+
+    print("case: batch", flush=True)
+    ri, _, _, w, h = S.info(5)
+    out = torch.zeros((h, w, 4), dtype=torch.uint8, device="cuda")
+    batch.render([(r, ri, out)])
+    S.check(out, 5, what="batch")
+    S.check(S.render(2), 2, what="plain after batch")
+
+    print("case: learned-start resolve with learned starts off", flush=True)
+    ri, _, _, w, h = S.info(3)
+    li = Ln.LearnedInit(w, h, 2)
+    out = torch.zeros((h, w, 4), dtype=torch.uint8, device="cuda")
+    li.render(r, out, ri, conf_threshold=1.0)
+    S.check(out, 3, what="learned")
+    # the prepared frame stays resolvable after its resolve
+    out2 = torch.zeros_like(out)
+    Ln.render_learned(r, out2, ri, li.out, 1.0)
+    torch.cuda.synchronize()
+    assert torch.equal(out, out2)
+    S.check(S.render(4, 1), 4, 1, what="window 3 after learned")
+
+    print("case: keyframe size change", flush=True)
+    cfg2 = F.scaled(2, 9, 5)
+    S2 = _Seq(oracle_lib, cfg2, r)
+    S2.check(S2.render(2), 2, what="new size 1")
+    S2.check(S2.render(3), 3, what="new size 2")
+    li.close()
+    batch.close()
+    r.close()
+
