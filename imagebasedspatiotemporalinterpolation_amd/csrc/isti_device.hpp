@@ -140,16 +140,21 @@ __device__ __forceinline__ float key_depth(unsigned long long k) { return __uint
 __device__ __forceinline__ uint32_t key_x(unsigned long long k) { return (uint32_t)k >> 16; }
 __device__ __forceinline__ uint32_t key_y(unsigned long long k) { return (uint32_t)(k & 0xFFFFull); }
 
-// Unchecked load at pixel (x, y) of a plane with row length w (32-bit index:
-// a view has fewer than 2^32 pixels).
+// Linear index of pixel (x, y) in a plane with row length w. Views are at
+// most 65535 pixels per axis, so y * w has 24-bit operands and a product
+// below 2^32: the full-rate 24-bit multiply (v_mad_u32_u24) is exact where
+// a 32-bit multiply would take four passes.
+__device__ __forceinline__ uint32_t px_index(uint32_t w, uint32_t x, uint32_t y) { return __umul24(y, w) + x; }
+
+// Unchecked load at pixel (x, y) of a plane with row length w.
 template <typename T>
 __device__ __forceinline__ T at(const T *plane, uint32_t w, uint32_t x, uint32_t y) {
-    return plane[y * w + x];
+    return plane[px_index(w, x, y)];
 }
 
 // The xy half of a float4 plane's sample (one 8-byte load).
 __device__ __forceinline__ float2 at_xy(const float4 *plane, uint32_t w, uint32_t x, uint32_t y) {
-    return reinterpret_cast<const float2 *>(plane)[2u * (y * w + x)];
+    return reinterpret_cast<const float2 *>(plane)[2u * px_index(w, x, y)];
 }
 
 // Point sample with border addressing (src/util/image_buffer.cu:227-249):
@@ -157,7 +162,7 @@ __device__ __forceinline__ float2 at_xy(const float4 *plane, uint32_t w, uint32_
 template <typename T>
 __device__ __forceinline__ T fetch(const T *plane, uint32_t w, uint32_t h, uint32_t x, uint32_t y) {
     if (plane == nullptr || x >= w || y >= h) return T{};
-    return plane[y * w + x];
+    return plane[px_index(w, x, y)];
 }
 
 }  // namespace isti

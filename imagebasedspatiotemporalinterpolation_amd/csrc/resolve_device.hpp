@@ -38,13 +38,14 @@ struct SplatInfo {
 // is the reference's `continue` for out-of-range cells.
 template <int W>
 __device__ __forceinline__ SplatInfo init_search_lds(const unsigned long long *t, int rw, int lx, int ly) {
-    const unsigned long long c = t[ly * rw + lx];
+    const unsigned long long *centre = t + (__umul24((uint32_t)ly, (uint32_t)rw) + (uint32_t)lx);  // ly, lx >= W
+    const unsigned long long c = centre[0];
     SplatInfo best{key_depth(c), key_x(c), key_y(c)};
 #pragma unroll
     for (int dy = -W; dy <= W; dy++) {
 #pragma unroll
         for (int dx = -W; dx <= W; dx++) {
-            const unsigned long long k = t[(ly + dy) * rw + lx + dx];
+            const unsigned long long k = centre[dy * rw + dx];
             const float d = key_depth(k);
             if (d > 0.0f && (!__builtin_isfinite(best.depth) || best.depth < 0.0f || d < best.depth))
                 best = SplatInfo{d, key_x(k), key_y(k)};
@@ -55,15 +56,21 @@ __device__ __forceinline__ SplatInfo init_search_lds(const unsigned long long *t
 
 // Stages the splat cells [ox, ox+rw) x [oy, oy+rh) of view v into LDS rows
 // of constant stride SPAN (rw <= SPAN).
+// The row of cell i is i / SPAN, taken as (i * M) >> 20 with M =
+// ceil(2^20 / SPAN) on the 24-bit multiplier: for i < 2^10 the excess
+// i (M - 2^20 / SPAN) / 2^20 < 2^-10 stays below 1 / SPAN (SPAN < 2^10), so
+// the quotient is exact.
 template <int SPAN>
 __device__ __forceinline__ void stage_splat(unsigned long long *dst, const ResolveView &v, int ox, int oy, int rw,
                                             int rh) {
-    const int n = SPAN * rh;
+    constexpr uint32_t M = ((1u << 20) + SPAN - 1) / SPAN;
+    static_assert(SPAN < 1024, "staged row too long for the 24-bit row split");
+    const int n = SPAN * rh;  // rh <= ROWS, and SPAN * ROWS < 2^10 (ResolveLds)
     for (int i = (int)threadIdx.x; i < n; i += (int)RESOLVE_THREADS) {
-        const int yy = i / SPAN, xx = i - yy * SPAN;
+        const int yy = (int)(__umul24((uint32_t)i, M) >> 20), xx = i - yy * SPAN;
         const int gx = ox + xx, gy = oy + yy;
         const bool in = gx >= 0 && gx < (int)v.w && gy >= 0 && gy < (int)v.h;
-        if (xx < rw) dst[i] = in ? v.splat[(uint32_t)gy * v.w + (uint32_t)gx] : SPLAT_EMPTY;
+        if (xx < rw) dst[i] = in ? v.splat[px_index(v.w, (uint32_t)gx, (uint32_t)gy)] : SPLAT_EMPTY;
     }
 }
 
@@ -86,7 +93,7 @@ __device__ __forceinline__ uint32_t pack_px(uint32_t x, uint32_t y) { return (y 
 // dependent gathers overlap instead of running one chain after another.
 struct Chain {
     uint32_t h0, h1, h2, h3;  // q_i, q_{i-1}, q_{i-2}, q_{i-3} (packed)
-    float2 m;                 // cam_ff.xy at h0
+    float2 m;                 // cam_ff.xy at h0 while the chain is active
     bool active;
 };
 constexpr uint32_t PX_NONE = 0xffffffffu;  // x, y <= 65534: never a packed pixel
@@ -118,8 +125,7 @@ __device__ __forceinline__ void chain_step(Chain &c, int i, int n_iter, float2 p
     if (p != 0) {
         const int idx = p - 1 - (n_iter - (i + 1)) % p;  // c_r = h_{p-1-r}
         const uint32_t q = idx == 0 ? c.h0 : (idx == 1 ? c.h1 : (idx == 2 ? c.h2 : c.h3));
-        c.h0 = q;
-        c.m = at_xy(v.cam_ff, v.w, q & 0xFFFFu, q >> 16);
+        c.h0 = q;  // the chain stops: m is not needed again
         c.active = false;
         return;
     }
@@ -309,6 +315,7 @@ struct ResolveLds {
     static constexpr int NC = 1 + NAUX;
     static constexpr int SPAN = TW + 2 * W;  // staged row stride (cells)
     static constexpr int ROWS = TH + 2 * W;  // staged rows
+    static_assert(SPAN * ROWS < 1024, "stage_splat splits cell indices below 2^10");
     unsigned long long splat[NC][2][SPAN * ROWS];
 };
 

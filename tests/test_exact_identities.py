@@ -75,6 +75,30 @@ def test_div_small_magic_multiply():
         assert np.array_equal(fast(x, int(r)), x // int(r)), r
 
 
+def test_staged_row_split_24_bit():
+    """The LDS staging splits cell index i into (i / SPAN, i % SPAN) as
+    (i * ceil(2^20 / SPAN)) >> 20 on the 24-bit multiplier: exact for every
+    i < 2^10 and every row stride 1 < SPAN < 2^10 (the product stays below
+    2^32 and both operands below 2^24)."""
+    i = np.arange(1024, dtype=np.int64)
+    for span in range(2, 1024):
+        m = ((1 << 20) + span - 1) // span
+        assert m < 2**24 and int(i[-1]) * m < 2**32
+        assert np.array_equal((i * m) >> 20, i // span), span
+
+
+def test_pixel_index_24_bit_products():
+    """y * w + x on the 24-bit multiplier is the 32-bit index for views up to
+    65535 pixels per axis: the product of two 16-bit values is below 2^32."""
+    rng = np.random.default_rng(5)
+    w = np.concatenate([rng.integers(1, 65536, 100_000), [1, 65535]]).astype(np.uint64)
+    y = (rng.integers(0, 2**16, w.shape).astype(np.uint64) % w)
+    low24 = np.uint64(0xFFFFFF)
+    prod24 = ((y & low24) * (w & low24)) & np.uint64(0xFFFFFFFF)  # v_mul_u32_u24: low 32 bits
+    assert np.array_equal(prod24, y * w)
+    assert int((y * w).max()) < 2**32
+
+
 def test_shadow_code_min_max_bit_parallel():
     """getMinMaxShadowValues (renderer.cu:217-225): the loop over the 2-bit
     codes of sub-frames 0..sfpa-1 against the kernels' bit-parallel form, for
