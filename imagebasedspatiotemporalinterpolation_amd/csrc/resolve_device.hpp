@@ -569,7 +569,17 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
     const Cand wk = use_aux ? aux_k : main_k;
     const SplatInfo wf = use_aux ? aux_f : si[0][0];
     const SplatInfo wb = use_aux ? aux_b : si[0][1];
-    const uchar4 col = cand_colour(wk, fc, wf, wb, a.fwd[wc], a.bwd[wc], wc == 0 ? EPS_2_FULL_RES : EPS_2_LOW_RES);
+    // One uniform branch per camera, so each camera's view descriptors stay
+    // scalar (a per-lane camera index would select every descriptor field
+    // per lane); a wave runs the branches its lanes chose.
+    uchar4 col = make_uchar4(0, 0, 0, 0);
+    if constexpr (NC == 1) {
+        col = cand_colour(wk, fc, wf, wb, a.fwd[0], a.bwd[0], EPS_2_FULL_RES);
+    } else {
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+            if (wc == c) col = cand_colour(wk, fc, wf, wb, a.fwd[c], a.bwd[c], c == 0 ? EPS_2_FULL_RES : EPS_2_LOW_RES);
+    }
     a.out[y * a.out_pitch_px + x] = col;
 }
 
