@@ -43,6 +43,7 @@ EXPORTED_SYMBOLS = (
     "isti_autotune",
     "isti_set_tile_shape",
     "isti_set_splat_binning",
+    "isti_selftest_division",
 )
 ISTI_N_TILE_SHAPES = 4
 ISTI_AUTOTUNE_TIMES = ISTI_N_TILE_SHAPES + 2
@@ -149,6 +150,8 @@ def lib() -> ctypes.CDLL:
     L.isti_last_error.restype = c_char_p
     L.isti_version.argtypes = []
     L.isti_version.restype = c_char_p
+    L.isti_selftest_division.argtypes = [c_int32, ctypes.c_uint64, ctypes.c_uint64, c_void_p, c_void_p]
+    L.isti_selftest_division.restype = c_int32
     L.isti_batch_create.argtypes = [c_int32, c_int32, POINTER(c_void_p)]
     L.isti_batch_create.restype = c_int32
     L.isti_batch_destroy.argtypes = [c_void_p]

@@ -998,6 +998,13 @@ const char *isti_last_error(const isti_renderer *r) {
     return r->err.c_str();
 }
 
+isti_status isti_selftest_division(int32_t device, uint64_t n, uint64_t seed, uint32_t *counts, void *stream) {
+    if (!counts) return ISTI_ERR_INVALID_ARG;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = launch_div_selftest(n, seed, counts, reinterpret_cast<hipStream_t>(stream));
+    return e == hipSuccess ? ISTI_OK : ISTI_ERR_DEVICE;
+}
+
 const char *isti_version(void) { return "isti-gfx950 0.1.0"; }
 
 }  // extern "C"

@@ -94,13 +94,45 @@ __device__ __forceinline__ float4 mat_vec(const Mat4 &M, float4 v) {
     r.w = (m[3] * v.x + m[7] * v.y) + (m[11] * v.z + m[15] * v.w);
     return r;
 }
+// Class masks of v_cmp_class_f32 (bit 2 -inf, 3 -normal, 8 +normal, 9 +inf).
+constexpr int CLASS_NORMAL = (1 << 3) | (1 << 8);
+constexpr int CLASS_NORMAL_OR_INF = CLASS_NORMAL | (1 << 2) | (1 << 9);
+
+// RN32(n / w) for the three numerators of a projective division, equal to
+// the IEEE quotients. For a normal w, R = 1/w comes from the hardware
+// reciprocal (error < 2^-22) refined by two Newton steps in binary64
+// (error < 2^-52), and q = RN32(RN64(n R)) is within 2^-51 |n / w| of the
+// exact quotient. In the normal range the exact quotient of two binary32
+// values is never a rounding midpoint M 2^f (M odd with 25 bits: n = w M 2^f
+// would need a 25-bit significand) and lies at least 2^f / B > 2^-49 |n / w|
+// from every midpoint (B < 2^24 the significand of w), so q rounds the same
+// way; the one subnormal midpoint that rounds up to 2^-126 is an exact
+// quotient only for w a power of two, where R and q are exact. So q is the
+// correctly rounded quotient whenever it is normal or infinite. A zero,
+// subnormal or NaN q (exact midpoints occur there, or the reciprocal
+// flushed) and a w that is not normal take the IEEE division.
+__device__ __forceinline__ float3 div3_rn(float x, float y, float z, float w) {
+    if (__builtin_amdgcn_classf(w, CLASS_NORMAL)) {
+        const double wd = (double)w;
+        double r = (double)__builtin_amdgcn_rcpf(w);
+        r = __builtin_fma(r, __builtin_fma(-wd, r, 1.0), r);
+        r = __builtin_fma(r, __builtin_fma(-wd, r, 1.0), r);
+        const float qx = (float)((double)x * r), qy = (float)((double)y * r), qz = (float)((double)z * r);
+        if (__builtin_amdgcn_classf(qx, CLASS_NORMAL_OR_INF) && __builtin_amdgcn_classf(qy, CLASS_NORMAL_OR_INF) &&
+            __builtin_amdgcn_classf(qz, CLASS_NORMAL_OR_INF))
+            return make_float3(qx, qy, qz);
+    }
+    return make_float3(x / w, y / w, z / w);
+}
+
 // helper_common.h:77-87 CS2WS / WS2CS. w/w is 1 for finite non-zero w and
 // NaN otherwise, so that quotient is a select instead of a division.
 __device__ __forceinline__ float4 project(const Mat4 &M, float4 v) {
     const float4 r = mat_vec(M, v);
     const float w = r.w;
     const float ww = (__builtin_isfinite(w) && w != 0.0f) ? 1.0f : __builtin_nanf("");
-    return make_float4(r.x / w, r.y / w, r.z / w, ww);
+    const float3 q = div3_rn(r.x, r.y, r.z, w);
+    return make_float4(q.x, q.y, q.z, ww);
 }
 // helper_common.h:89-95
 __device__ __forceinline__ float z_to_depth(float z) { return (z + 1.0f) * 0.5f; }

@@ -555,6 +555,75 @@ hipError_t launch_mean_z(const float *z, uint32_t w, uint32_t h, double *row_sum
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Self-test of div3_rn against the IEEE division. Operand families (i mod 4):
+// 0 random bit patterns (every class), 1 moderate exponents [-40, 40],
+// 2 quotients near rounding midpoints (n = RN32((q + ulp(q)/2) w)),
+// 3 every exponent field including subnormals. Mismatching bit patterns are
+// counted per family (counts[0..3]), and so are the sets whose quotients came
+// from the reciprocal path (counts[4..7]).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t mix64(uint64_t v) {  // splitmix64 finaliser
+    v ^= v >> 30;
+    v *= 0xbf58476d1ce4e5b9ull;
+    v ^= v >> 27;
+    v *= 0x94d049bb133111ebull;
+    return v ^ (v >> 31);
+}
+__device__ __forceinline__ float moderate(uint32_t r, int span) {
+    const uint32_t e = 127u + (uint32_t)((int)((r >> 23) % (uint32_t)(2 * span + 1)) - span);
+    return __uint_as_float((r & 0x80000000u) | (e << 23) | (r & 0x7fffffu));
+}
+__device__ __forceinline__ float near_midpoint(uint32_t r, float w) {
+    const float q = moderate(r, 20);
+    const double m = (double)q + ldexp((double)(q < 0.0f ? -1.0 : 1.0), ilogbf(q) - 24);
+    return (float)(m * (double)w);
+}
+__global__ __launch_bounds__(256) void div_selftest_kernel(uint64_t n, uint64_t seed, uint32_t *counts) {
+    const uint64_t stride = (uint64_t)gridDim.x * 256u;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += stride) {
+        const uint64_t h0 = mix64(seed + 0x9e3779b97f4a7c15ull * (i + 1)), h1 = mix64(h0);
+        const uint32_t r0 = (uint32_t)h0, r1 = (uint32_t)(h0 >> 32), r2 = (uint32_t)h1, r3 = (uint32_t)(h1 >> 32);
+        const int fam = (int)(i & 3u);
+        float x, y, z, w;
+        if (fam == 0) {
+            x = __uint_as_float(r0), y = __uint_as_float(r1), z = __uint_as_float(r2), w = __uint_as_float(r3);
+        } else if (fam == 1) {
+            x = moderate(r0, 40), y = moderate(r1, 40), z = moderate(r2, 40), w = moderate(r3, 40);
+        } else if (fam == 2) {
+            w = moderate(r3, 20);
+            x = near_midpoint(r0, w), y = near_midpoint(r1, w), z = near_midpoint(r2, w);
+        } else {
+            x = __uint_as_float(r0 & 0x807fffffu | (((r0 >> 8) % 255u) << 23));
+            y = __uint_as_float(r1 & 0x807fffffu | (((r1 >> 8) % 255u) << 23));
+            z = __uint_as_float(r2 & 0x807fffffu | (((r2 >> 8) % 255u) << 23));
+            w = __uint_as_float(r3 & 0x807fffffu | (((r3 >> 8) % 255u) << 23));
+        }
+        const float3 f = div3_rn(x, y, z, w);
+        const bool bad = __float_as_uint(f.x) != __float_as_uint(x / w) ||
+                         __float_as_uint(f.y) != __float_as_uint(y / w) ||
+                         __float_as_uint(f.z) != __float_as_uint(z / w);
+        if (bad) atomicAdd(&counts[fam], 1u);
+        // sets the reciprocal path answered (the rest fell back to the division)
+        const bool fast = __builtin_amdgcn_classf(w, CLASS_NORMAL) &&
+                          __builtin_amdgcn_classf(f.x, CLASS_NORMAL_OR_INF) &&
+                          __builtin_amdgcn_classf(f.y, CLASS_NORMAL_OR_INF) &&
+                          __builtin_amdgcn_classf(f.z, CLASS_NORMAL_OR_INF);
+        const bool leader = (int)(threadIdx.x & 63u) == __ffsll((long long)__ballot(1)) - 1;
+        for (int k = 0; k < 4; k++) {  // lanes of one wave mix the families
+            const unsigned long long lanes = __ballot(fast && fam == k);
+            if (leader && lanes != 0) atomicAdd(&counts[4 + k], (uint32_t)__popcll(lanes));
+        }
+    }
+}
+
+hipError_t launch_div_selftest(uint64_t n, uint64_t seed, uint32_t *counts, hipStream_t s) {
+    const uint64_t blocks = (n + 255u) / 256u;
+    const uint32_t grid = (uint32_t)(blocks < 8192u ? (blocks > 0 ? blocks : 1u) : 8192u);
+    hipLaunchKernelGGL(div_selftest_kernel, dim3(grid), dim3(256), 0, s, n, seed, counts);
+    return hipGetLastError();
+}
+
 hipError_t launch_timewarp(const TimewarpArgs &a, hipStream_t s) {
     if (a.n_tiles == 0) return hipSuccess;
     hipLaunchKernelGGL(timewarp_kernel, dim3(a.n_tiles), dim3(TILE_W * TILE_H), 0, s, a);

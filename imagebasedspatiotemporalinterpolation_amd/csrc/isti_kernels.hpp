@@ -166,5 +166,6 @@ uint32_t splat_blocks(uint32_t w, uint32_t h, bool binned);
 hipError_t launch_resolve(const ResolveArgs &a, int window, bool always_valid, int shape, hipStream_t s);
 hipError_t launch_mean_z(const float *z, uint32_t w, uint32_t h, double *row_sums, float *mean, hipStream_t s);
 hipError_t launch_timewarp(const TimewarpArgs &a, hipStream_t s);
+hipError_t launch_div_selftest(uint64_t n, uint64_t seed, uint32_t *counts, hipStream_t s);
 
 }  // namespace isti

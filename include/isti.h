@@ -254,6 +254,16 @@ const char *isti_last_error(const isti_renderer *r);
 /* Library version string. */
 const char *isti_version(void);
 
+/* Diagnostic with no counterpart in the reference: checks the projective
+ * division of the reprojection (RN32(n / w) of three numerators through one
+ * refined reciprocal of w, exact by construction) against the IEEE division
+ * on n generated operand sets of four families: random bit patterns, moderate
+ * exponents, quotients next to rounding midpoints, every exponent including
+ * subnormals. counts: device uint32[8], accumulated (zero it first):
+ * [0..3] mismatches per family, which stay 0; [4..7] the sets per family
+ * answered by the reciprocal path rather than the IEEE fallback. */
+isti_status isti_selftest_division(int32_t device, uint64_t n, uint64_t seed, uint32_t *counts, void *stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif
