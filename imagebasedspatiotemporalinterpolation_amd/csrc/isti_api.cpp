@@ -809,6 +809,8 @@ isti_status isti_keyframe_set(isti_renderer *r, int32_t slot, int32_t view, cons
         return fail(r, ISTI_ERR_INVALID_ARG, "slot must be 0/1 and view 0..n_aux, desc non-null");
     if (d->width == 0 || d->height == 0 || d->width > ISTI_MAX_DIM || d->height > ISTI_MAX_DIM)
         return fail(r, ISTI_ERR_UNSUPPORTED_SHAPE, "view size must be 1..65535 per axis");
+    if ((uint64_t)d->width * d->height > ISTI_MAX_VIEW_PIXELS)
+        return fail(r, ISTI_ERR_UNSUPPORTED_SHAPE, "a view holds at most 2^28 - 1 pixels");
     if (!d->color || !d->z) return fail(r, ISTI_ERR_INVALID_ARG, "color and z planes are required");
     const int nm = (d->motion_x != nullptr) + (d->motion_y != nullptr) + (d->motion_z != nullptr);
     if (nm != 0 && nm != 3) return fail(r, ISTI_ERR_INVALID_ARG, "motion planes: all three or none");

@@ -178,15 +178,24 @@ __device__ __forceinline__ uint32_t key_y(unsigned long long k) { return (uint32
 // a 32-bit multiply would take four passes.
 __device__ __forceinline__ uint32_t px_index(uint32_t w, uint32_t x, uint32_t y) { return __umul24(y, w) + x; }
 
+// Element idx of a plane through a 32-bit byte offset: views hold fewer than
+// 2^28 pixels (ISTI_MAX_VIEW_PIXELS), so even float4 planes stay below 4 GiB,
+// and the load takes the scalar base + 32-bit vector offset form (no 64-bit
+// address arithmetic, one VGPR per address).
+template <typename T>
+__device__ __forceinline__ const T &plane_ref(const T *plane, uint32_t idx) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(plane) + idx * (uint32_t)sizeof(T));
+}
+
 // Unchecked load at pixel (x, y) of a plane with row length w.
 template <typename T>
 __device__ __forceinline__ T at(const T *plane, uint32_t w, uint32_t x, uint32_t y) {
-    return plane[px_index(w, x, y)];
+    return plane_ref(plane, px_index(w, x, y));
 }
 
 // The xy half of a float4 plane's sample (one 8-byte load).
 __device__ __forceinline__ float2 at_xy(const float4 *plane, uint32_t w, uint32_t x, uint32_t y) {
-    return reinterpret_cast<const float2 *>(plane)[2u * px_index(w, x, y)];
+    return plane_ref(reinterpret_cast<const float2 *>(plane), 2u * px_index(w, x, y));
 }
 
 // Point sample with border addressing (src/util/image_buffer.cu:227-249):
@@ -194,7 +203,7 @@ __device__ __forceinline__ float2 at_xy(const float4 *plane, uint32_t w, uint32_
 template <typename T>
 __device__ __forceinline__ T fetch(const T *plane, uint32_t w, uint32_t h, uint32_t x, uint32_t y) {
     if (plane == nullptr || x >= w || y >= h) return T{};
-    return plane[px_index(w, x, y)];
+    return plane_ref(plane, px_index(w, x, y));
 }
 
 }  // namespace isti

@@ -70,7 +70,7 @@ __device__ __forceinline__ void stage_splat(unsigned long long *dst, const Resol
         const int yy = (int)(__umul24((uint32_t)i, M) >> 20), xx = i - yy * SPAN;
         const int gx = ox + xx, gy = oy + yy;
         const bool in = gx >= 0 && gx < (int)v.w && gy >= 0 && gy < (int)v.h;
-        if (xx < rw) dst[i] = in ? v.splat[px_index(v.w, (uint32_t)gx, (uint32_t)gy)] : SPLAT_EMPTY;
+        if (xx < rw) dst[i] = in ? plane_ref(v.splat, px_index(v.w, (uint32_t)gx, (uint32_t)gy)) : SPLAT_EMPTY;
     }
 }
 

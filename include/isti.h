@@ -48,6 +48,9 @@ extern "C" {
 /* Largest view width/height: source pixels are packed as 16-bit coordinates
  * into the splat keys (src/renderer.cu:65-68). */
 #define ISTI_MAX_DIM 65535
+/* Largest width * height of a view (2^28 - 1): byte offsets into its float4
+ * flow planes fit 32 bits. */
+#define ISTI_MAX_VIEW_PIXELS 268435455u
 /* Largest shadow_frames_per_anchor: 2-bit codes in a 24-bit field. */
 #define ISTI_MAX_SHADOW_FRAMES 12
 /* Upper bound of isti_render_info.max_search_iterations. */

@@ -169,3 +169,27 @@ def test_projective_division_is_ieee():
     # entirely, and part of the other two (the rest is the fallback)
     assert c[5] == per_family and c[6] == per_family
     assert 0 < c[4] < per_family and 0 < c[7] < per_family
+
+
+def test_view_size_limits_are_rejected():
+    """Keyframe views hold at most 65535 pixels per axis and 2^28 - 1 pixels
+    in all (32-bit byte offsets into the float4 flow planes); larger views
+    are refused before any device work."""
+    import ctypes
+
+    from imagebasedspatiotemporalinterpolation_amd import _native as N
+    from imagebasedspatiotemporalinterpolation_amd import renderer as R
+
+    r = R.Renderer(n_aux=0)
+    L = N.lib()
+    dummy = 16  # never dereferenced: the size checks come first
+    for w, h in ((65535, 4097), (65536, 1), (0, 5)):
+        d = N.ViewDesc()
+        d.width, d.height = w, h
+        d.color, d.z = dummy, dummy
+        assert L.isti_keyframe_set(r._h, 0, 0, ctypes.byref(d)) == N.ISTI_ERR_UNSUPPORTED_SHAPE, (w, h)
+    d = N.ViewDesc()
+    d.width, d.height = 65535, 4096  # 268,431,360 pixels: allowed
+    d.color, d.z = dummy, dummy
+    assert L.isti_keyframe_set(r._h, 0, 0, ctypes.byref(d)) == N.ISTI_OK
+    r.close()
