@@ -82,14 +82,14 @@ __device__ __forceinline__ uint16_t load_src1(const Src &s, int n, int y, int x,
 }
 
 // out[n][y][x][co] = act(bias[co] + sum_{ky,kx,ci} w[co][ky][kx][ci] * in[n][y+ky-p][x+kx-p][ci])
-// A wave computes MT groups of 16 consecutive pixels of a row x NT*16
-// outputs: each weight fragment loaded for a K step feeds MT MFMAs and each
-// input fragment NT MFMAs (register blocking; the K order per output, and so
-// the result, does not depend on MT).
+// A wave computes MT groups of 16 consecutive pixels of a row x NT*16 of the
+// cout outputs, the group blockIdx.y: each weight fragment loaded for a K step
+// feeds MT MFMAs and each input fragment NT MFMAs (register blocking; the K
+// order per output, and so the result, depends on neither MT nor NT).
 template <int NT, int MT>
 __global__ __launch_bounds__(256) void conv_kernel(Src s, const uint16_t *__restrict__ w, const float *__restrict__ bias,
-                                                   int cin, int ks, int N, int H, int W, uint16_t *out_bf, float *out_f,
-                                                   int relu) {
+                                                   int cin, int ks, int N, int H, int W, int cout, uint16_t *out_bf,
+                                                   float *out_f, int relu) {
     const int lane = threadIdx.x & 63;
     constexpr int PXW = 16 * MT;
     const int chunks = (W + PXW - 1) / PXW;
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void conv_kernel(Src s, const uint16_t *__rest
     const int y = rem / chunks, x0 = (rem - y * chunks) * PXW;
     const int pad = ks >> 1;
     const int kq = (lane >> 4) * 4;
-    constexpr int COUT = NT * 16;
+    const int co_base = (int)blockIdx.y * NT * 16;
     f32x4 acc[MT][NT];
 #pragma unroll
     for (int m = 0; m < MT; m++)
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256) void conv_kernel(Src s, const uint16_t *__rest
                 xx[m] = px + kx - pad;
                 in[m] = yy >= 0 && yy < H && xx[m] >= 0 && xx[m] < W && px < W;
             }
-            const uint16_t *wp0 = w + (size_t)((lane & 15) * ks * ks + ky * ks + kx) * cin + kq;
+            const uint16_t *wp0 = w + (size_t)((co_base + (lane & 15)) * ks * ks + ky * ks + kx) * cin + kq;
             for (int c0 = 0; c0 < cin; c0 += 16) {
                 bf16x4 av[MT];
 #pragma unroll
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(256) void conv_kernel(Src s, const uint16_t *__rest
     }
 #pragma unroll
     for (int t = 0; t < NT; t++) {
-        const int co = t * 16 + (lane & 15);
+        const int co = co_base + t * 16 + (lane & 15);
         const float bsv = bias ? bias[co] : 0.0f;
 #pragma unroll
         for (int m = 0; m < MT; m++) {
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(256) void conv_kernel(Src s, const uint16_t *__rest
                 if (x >= W) continue;
                 float v = acc[m][t][i] + bsv;
                 if (relu) v = v > 0.0f ? v : 0.0f;
-                const size_t o = (((size_t)n * H + y) * W + x) * COUT + co;
+                const size_t o = (((size_t)n * H + y) * W + x) * cout + co;
                 if (out_f) out_f[o] = v;
                 else out_bf[o] = f2bf(v);
             }
@@ -356,38 +356,53 @@ unsigned blocks_for(long n, K) {
     return (unsigned)((n + 255) / 256);
 }
 
-// Pixel groups per wave: wide rows get 4 (or 2) groups while the
-// accumulators (MT * NT tiles) stay within 16 tiles.
-template <int NT>
-hipError_t launch_conv_nt(const Src &s, const uint16_t *w, const float *bias, int cin, int ks, int N, int H, int W,
-                          uint16_t *out_bf, float *out_f, int relu, hipStream_t st) {
-    constexpr int MT_MAX = NT <= 4 ? 4 : (NT <= 8 ? 2 : 1);
-    const int mt = W >= 192 ? MT_MAX : (W >= 64 ? (MT_MAX < 2 ? MT_MAX : 2) : 1);
-    const long waves = (long)N * H * ((W + 16 * mt - 1) / (16 * mt));
-    const dim3 grid((unsigned)((waves + 3) / 4)), block(256);
-    if (mt == 4)
-        hipLaunchKernelGGL((conv_kernel<NT, (MT_MAX >= 4 ? 4 : 1)>), grid, block, 0, st, s, w, bias, cin, ks, N, H, W,
-                           out_bf, out_f, relu);
-    else if (mt == 2)
-        hipLaunchKernelGGL((conv_kernel<NT, (MT_MAX >= 2 ? 2 : 1)>), grid, block, 0, st, s, w, bias, cin, ks, N, H, W,
-                           out_bf, out_f, relu);
-    else
-        hipLaunchKernelGGL((conv_kernel<NT, 1>), grid, block, 0, st, s, w, bias, cin, ks, N, H, W, out_bf, out_f,
-                           relu);
+template <int NT, int MT>
+hipError_t launch_conv_tiles(const Src &s, const uint16_t *w, const float *bias, int cin, int cout, int ks, int N, int H,
+                             int W, uint16_t *out_bf, float *out_f, int relu, hipStream_t st) {
+    const long waves = (long)N * H * ((W + 16 * MT - 1) / (16 * MT));
+    const dim3 grid((unsigned)((waves + 3) / 4), (unsigned)(cout / (16 * NT))), block(256);
+    hipLaunchKernelGGL((conv_kernel<NT, MT>), grid, block, 0, st, s, w, bias, cin, ks, N, H, W, cout, out_bf, out_f,
+                       relu);
     return hipGetLastError();
 }
 
+// Wave shape (NT output tiles x MT pixel groups): the most register
+// blocking that still gives the launch CONV_MIN_WAVES waves (2 per SIMD of
+// the chip), else the most waves. Deep levels are small (60 x 34 pixels),
+// so their output channels are split over the grid's y dimension.
+constexpr long CONV_MIN_WAVES = 2048;
+
 hipError_t launch_conv(const Src &s, const uint16_t *w, const float *bias, int cin, int cout, int ks, int N, int H,
                        int W, uint16_t *out_bf, float *out_f, int relu, hipStream_t st) {
-    switch (cout / 16) {
-        case 1: return launch_conv_nt<1>(s, w, bias, cin, ks, N, H, W, out_bf, out_f, relu, st);
-        case 2: return launch_conv_nt<2>(s, w, bias, cin, ks, N, H, W, out_bf, out_f, relu, st);
-        case 3: return launch_conv_nt<3>(s, w, bias, cin, ks, N, H, W, out_bf, out_f, relu, st);
-        case 4: return launch_conv_nt<4>(s, w, bias, cin, ks, N, H, W, out_bf, out_f, relu, st);
-        case 6: return launch_conv_nt<6>(s, w, bias, cin, ks, N, H, W, out_bf, out_f, relu, st);
-        case 8: return launch_conv_nt<8>(s, w, bias, cin, ks, N, H, W, out_bf, out_f, relu, st);
-        case 12: return launch_conv_nt<12>(s, w, bias, cin, ks, N, H, W, out_bf, out_f, relu, st);
-        default: return hipErrorInvalidValue;
+    const int tiles = cout / 16;
+    if (tiles < 1 || tiles * 16 != cout) return hipErrorInvalidValue;
+    static const int shapes[][2] = {{4, 4}, {4, 2}, {2, 4}, {3, 2}, {4, 1}, {2, 2}, {1, 4}, {3, 1},
+                                    {2, 1}, {1, 2}, {1, 1}};
+    int pick = -1, most = -1;
+    long most_waves = -1;
+    for (int i = 0; i < (int)(sizeof(shapes) / sizeof(shapes[0])); i++) {
+        const int nt = shapes[i][0], mt = shapes[i][1];
+        if (tiles % nt != 0 || (mt == 4 && W < 192) || (mt == 2 && W < 64)) continue;
+        const long waves = (long)N * H * ((W + 16 * mt - 1) / (16 * mt)) * (tiles / nt);
+        if (waves >= CONV_MIN_WAVES) {
+            pick = i;
+            break;
+        }
+        if (waves > most_waves) most_waves = waves, most = i;
+    }
+    if (pick < 0) pick = most;
+    switch (shapes[pick][0] * 8 + shapes[pick][1]) {
+        case 4 * 8 + 4: return launch_conv_tiles<4, 4>(s, w, bias, cin, cout, ks, N, H, W, out_bf, out_f, relu, st);
+        case 4 * 8 + 2: return launch_conv_tiles<4, 2>(s, w, bias, cin, cout, ks, N, H, W, out_bf, out_f, relu, st);
+        case 2 * 8 + 4: return launch_conv_tiles<2, 4>(s, w, bias, cin, cout, ks, N, H, W, out_bf, out_f, relu, st);
+        case 3 * 8 + 2: return launch_conv_tiles<3, 2>(s, w, bias, cin, cout, ks, N, H, W, out_bf, out_f, relu, st);
+        case 4 * 8 + 1: return launch_conv_tiles<4, 1>(s, w, bias, cin, cout, ks, N, H, W, out_bf, out_f, relu, st);
+        case 2 * 8 + 2: return launch_conv_tiles<2, 2>(s, w, bias, cin, cout, ks, N, H, W, out_bf, out_f, relu, st);
+        case 1 * 8 + 4: return launch_conv_tiles<1, 4>(s, w, bias, cin, cout, ks, N, H, W, out_bf, out_f, relu, st);
+        case 3 * 8 + 1: return launch_conv_tiles<3, 1>(s, w, bias, cin, cout, ks, N, H, W, out_bf, out_f, relu, st);
+        case 2 * 8 + 1: return launch_conv_tiles<2, 1>(s, w, bias, cin, cout, ks, N, H, W, out_bf, out_f, relu, st);
+        case 1 * 8 + 2: return launch_conv_tiles<1, 2>(s, w, bias, cin, cout, ks, N, H, W, out_bf, out_f, relu, st);
+        default: return launch_conv_tiles<1, 1>(s, w, bias, cin, cout, ks, N, H, W, out_bf, out_f, relu, st);
     }
 }
 
