@@ -74,6 +74,7 @@ struct isti_renderer {
     bool timing = false;
     TimingRing ring;
     uint32_t *q_out = nullptr;  // optional search-result capture (isti_set_search_capture)
+    uint32_t *stat_slots = nullptr;  // learned-start statistic rows (zero between frames)
     // frame whose splat stage isti_learned_features ran last (isti_render_learned resolves it)
     isti_render_info prepared{};
     bool prepared_valid = false;
@@ -745,8 +746,14 @@ isti_status isti_render_learned(isti_renderer *r, const isti_render_info *info, 
     // confidence threshold as a logit; 1 disables every learned start
     P.ra.conf_logit = conf_threshold < 1.0f ? std::log(conf_threshold / (1.0f - conf_threshold))
                                             : std::numeric_limits<float>::infinity();
-    P.ra.learned_stats = stats;
+    if (stats && !r->stat_slots) {  // first frame with statistics: zeroed slot rows
+        const size_t bytes = (size_t)STAT_SLOTS * STAT_STRIDE * sizeof(uint32_t);
+        if ((e = hipMalloc(&r->stat_slots, bytes)) != hipSuccess) return dev_fail(r, e, "hipMalloc(stat slots)");
+        if ((e = hipMemsetAsync(r->stat_slots, 0, bytes, s)) != hipSuccess) return dev_fail(r, e, "hipMemsetAsync(stat slots)");
+    }
+    P.ra.learned_stats = stats ? r->stat_slots : nullptr;
     if ((e = mark(r, 2, s)) == hipSuccess) e = launch_resolve_learned(P.ra, s);
+    if (e == hipSuccess && stats) e = launch_stats_reduce(r->stat_slots, stats, s);
     if (e == hipSuccess) e = mark(r, 3, s);
     return e == hipSuccess ? ISTI_OK : dev_fail(r, e, "learned render launch");
 }
@@ -795,6 +802,7 @@ isti_status isti_destroy(isti_renderer *r) {
         for (auto &s : slot) free_scratch(s);
     if (r->row_sums) (void)hipFree(r->row_sums);
     if (r->mean_z) (void)hipFree(r->mean_z);
+    if (r->stat_slots) (void)hipFree(r->stat_slots);
     for (auto &set : r->ring.ev)
         for (auto &ev : set)
             if (ev) (void)hipEventDestroy(ev);

@@ -555,6 +555,27 @@ hipError_t launch_mean_z(const float *z, uint32_t w, uint32_t h, double *row_sum
     return hipGetLastError();
 }
 
+// One block: thread t takes slot row t, the rows are summed through LDS into
+// the caller's counters and zeroed.
+__global__ __launch_bounds__(STAT_SLOTS) void stats_reduce_kernel(uint32_t *slots, uint32_t *out) {
+    __shared__ uint32_t sum[STAT_COUNTERS];
+    if (threadIdx.x < STAT_COUNTERS) sum[threadIdx.x] = 0u;
+    __syncthreads();
+    uint32_t *row = slots + threadIdx.x * STAT_STRIDE;
+    for (uint32_t k = 0; k < STAT_COUNTERS; k++) {
+        const uint32_t v = row[k];
+        row[k] = 0u;
+        if (v != 0u) atomicAdd(&sum[k], v);
+    }
+    __syncthreads();
+    if (threadIdx.x < STAT_COUNTERS) atomicAdd(&out[threadIdx.x], sum[threadIdx.x]);
+}
+
+hipError_t launch_stats_reduce(uint32_t *slots, uint32_t *out, hipStream_t s) {
+    hipLaunchKernelGGL(stats_reduce_kernel, dim3(1), dim3(STAT_SLOTS), 0, s, slots, out);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Self-test of div3_rn against the IEEE division. Operand families (i mod 4):
 // 0 random bit patterns (every class), 1 moderate exponents [-40, 40],

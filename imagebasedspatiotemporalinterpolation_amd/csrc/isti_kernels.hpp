@@ -157,6 +157,12 @@ hipError_t launch_learned_loss(const float *out, const uint32_t *q, const float4
                                uint32_t w, uint32_t h, uint32_t net_w, uint32_t net_h, uint32_t shift,
                                float conf_weight, float *dout, float *loss_acc, hipStream_t s);
 hipError_t launch_resolve_learned(const ResolveArgs &a, hipStream_t s);
+// Learned-start statistics: the resolve adds into STAT_SLOTS spread counter
+// rows (a row of STAT_STRIDE uint32 per slot; same-address atomics from
+// every wave would serialise at the memory side), then one block sums the
+// rows into the caller's five counters and zeroes them for the next frame.
+constexpr uint32_t STAT_SLOTS = 256, STAT_STRIDE = 8, STAT_COUNTERS = 5;
+hipError_t launch_stats_reduce(uint32_t *slots, uint32_t *out, hipStream_t s);
 
 hipError_t launch_clear(const ClearArgs &a, hipStream_t s);
 // binned: the LDS-binned variant; its blocks are 32x8 source tiles, so a

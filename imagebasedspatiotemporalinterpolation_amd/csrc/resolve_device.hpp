@@ -500,7 +500,7 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
                 vis[0][d] = m.w > 0.0f;
             }
         }
-        if (a.learned_stats) {  // one atomic per wave and counter (ballots see active lanes only)
+        if (a.learned_stats) {  // one atomic per wave and counter into the block's slot row (ballots see active lanes only)
             const bool fwd_ok = fc.alpha < 1.0f;
             const uint32_t kept = (uint32_t)(__popcll(__ballot(learned[0] && !redo[0])) +
                                              __popcll(__ballot(learned[1] && !redo[1])));
@@ -512,11 +512,12 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
             for (int b = 0; b < 16; b++) wave_steps += (uint32_t)__popcll(__ballot((steps >> b) & 1u)) << b;
             const unsigned long long act = __ballot(1);
             if ((int)(threadIdx.x & 63) == __ffsll((long long)act) - 1) {
-                atomicAdd(&a.learned_stats[0], kept);
-                atomicAdd(&a.learned_stats[1], rej);
-                atomicAdd(&a.learned_stats[2], low);
-                atomicAdd(&a.learned_stats[3], tot);
-                atomicAdd(&a.learned_stats[4], wave_steps);
+                uint32_t *row = a.learned_stats + (blockIdx.x % STAT_SLOTS) * STAT_STRIDE;
+                atomicAdd(&row[0], kept);
+                atomicAdd(&row[1], rej);
+                atomicAdd(&row[2], low);
+                atomicAdd(&row[3], tot);
+                atomicAdd(&row[4], wave_steps);
             }
         }
     }
