@@ -336,7 +336,7 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
     // Stage every splat cell the tile's windows read: camera c covers camera
     // pixels [x0/rx, xl/rx] (xl = last in-image column of the tile) plus a
     // W-cell border, at most SPAN x SPAN cells.
-    int org_x[NC], org_y[NC], span_x[NC];
+    int org_x[NC], org_y[NC], span_x[NC], span_y[NC];
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         const ResolveView &v = a.fwd[c];
@@ -345,11 +345,54 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
         org_x[c] = (int)cx0 - W;
         org_y[c] = (int)cy0 - W;
         span_x[c] = (int)(div_small(xl, v.ratio_x, v.magic_x) - cx0) + 1 + 2 * W;
-        const int span_y = (int)(div_small(yl, v.ratio_y, v.magic_y) - cy0) + 1 + 2 * W;
-        stage_splat<SPAN>(s_splat[c][0], a.fwd[c], org_x[c], org_y[c], span_x[c], span_y);
-        stage_splat<SPAN>(s_splat[c][1], a.bwd[c], org_x[c], org_y[c], span_x[c], span_y);
+        span_y[c] = (int)(div_small(yl, v.ratio_y, v.magic_y) - cy0) + 1 + 2 * W;
+        stage_splat<SPAN>(s_splat[c][0], a.fwd[c], org_x[c], org_y[c], span_x[c], span_y[c]);
+        stage_splat<SPAN>(s_splat[c][1], a.bwd[c], org_x[c], org_y[c], span_x[c], span_y[c]);
     }
     __syncthreads();
+    // 1a. aux cameras: a window start depends only on the aux pixel, which
+    // ratio_x * ratio_y render pixels share, so each distinct aux pixel of the
+    // tile is initialised once and its start replaces the staged cells (packed
+    // like a splat key) for the pixels to read.
+    if constexpr (NAUX > 0) {
+        constexpr int PER = (NAUX * 2 * TW * TH + (int)RESOLVE_THREADS - 1) / (int)RESOLVE_THREADS;
+        unsigned long long res[PER];
+        int nx[NC], cnt[NC], first[NC + 1];
+        first[1] = 0;
+#pragma unroll
+        for (int c = 1; c < NC; c++) {
+            nx[c] = span_x[c] - 2 * W;
+            cnt[c] = nx[c] * (span_y[c] - 2 * W);
+            first[c + 1] = first[c] + 2 * cnt[c];
+        }
+#pragma unroll
+        for (int r = 0; r < PER; r++) {
+            const int k = (int)threadIdx.x + r * (int)RESOLVE_THREADS;
+            res[r] = SPLAT_EMPTY;
+#pragma unroll
+            for (int c = 1; c < NC; c++) {
+                if (k >= first[c] && k < first[c + 1]) {
+                    const int j = k - first[c], d = j >= cnt[c] ? 1 : 0, p = j - d * cnt[c];
+                    const int py = p / nx[c], px = p - py * nx[c];
+                    const SplatInfo w = init_search_lds<W>(s_splat[c][d], SPAN, px + W, py + W);
+                    res[r] = encode_key(w.depth, w.x, w.y);
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < PER; r++) {
+            const int k = (int)threadIdx.x + r * (int)RESOLVE_THREADS;
+#pragma unroll
+            for (int c = 1; c < NC; c++) {
+                if (k >= first[c] && k < first[c + 1]) {
+                    const int j = k - first[c], d = j >= cnt[c] ? 1 : 0;
+                    s_splat[c][d][j - d * cnt[c]] = res[r];
+                }
+            }
+        }
+        __syncthreads();
+    }
     if (x >= a.w || y >= a.h) return;
 
     FrameConst fc;
@@ -377,11 +420,16 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
 
     // 1. window initialisation from LDS
     SplatInfo si[NC][2];
+    si[0][0] = init_search_lds<W>(s_splat[0][0], SPAN, (int)x - org_x[0], (int)y - org_y[0]);
+    si[0][1] = init_search_lds<W>(s_splat[0][1], SPAN, (int)x - org_x[0], (int)y - org_y[0]);
 #pragma unroll
-    for (int c = 0; c < NC; c++) {
-        const int lx = (int)cx[c] - org_x[c], ly = (int)cy[c] - org_y[c];
-        si[c][0] = init_search_lds<W>(s_splat[c][0], SPAN, lx, ly);
-        si[c][1] = init_search_lds<W>(s_splat[c][1], SPAN, lx, ly);
+    for (int c = 1; c < NC; c++) {
+        const int p = ((int)cy[c] - org_y[c] - W) * (span_x[c] - 2 * W) + ((int)cx[c] - org_x[c] - W);
+#pragma unroll
+        for (int d = 0; d < 2; d++) {
+            const unsigned long long k = s_splat[c][d][p];
+            si[c][d] = SplatInfo{key_depth(k), key_x(k), key_y(k)};
+        }
     }
     // 2. searches in lock-step; the fwd search is skipped for alpha >= 1 (:335)
     Chain ch[NC][2];
