@@ -44,7 +44,7 @@ struct Scratch {
     uint32_t w = 0, h = 0;
     unsigned long long *splat = nullptr;
     float4 *cam_ff = nullptr;
-    float4 *full_ff = nullptr;
+    Flow3 *full_ff = nullptr;
 };
 
 // Ring of per-render event sets for stage timing. A slot is harvested (its
@@ -108,7 +108,7 @@ isti_status ensure_scratch(isti_renderer *r, Scratch &s, uint32_t w, uint32_t h)
     const size_t n = (size_t)w * h;
     hipError_t e = hipMalloc(&s.splat, n * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc(&s.cam_ff, n * sizeof(float4));
-    if (e == hipSuccess) e = hipMalloc(&s.full_ff, n * sizeof(float4));
+    if (e == hipSuccess) e = hipMalloc(&s.full_ff, n * sizeof(Flow3));
     if (e != hipSuccess) {
         free_scratch(s);
         return dev_fail(r, e, "hipMalloc(scratch)");

@@ -75,7 +75,7 @@ __device__ __forceinline__ bool project_pixel(const J_t &J, const M_t &vp_cam, c
                                          ws_from.z + wt * (ws_to.z - ws_from.z), ws_from.w + wt * (ws_to.w - ws_from.w));
     const float4 cs_to = project(J.vp_to, ws_to);
     const float fx = cs_to.x - cs_from.x, fy = cs_to.y - cs_from.y, fz = cs_to.z - cs_from.z;
-    J.full_ff[p] = make_float4(fx, fy, fz, vis);
+    J.full_ff[p] = Flow3{fx, fy, fz};
 
     float depth = z_to_depth(cs_from.z);
     tx = x;

@@ -24,11 +24,18 @@ constexpr int N_TILE_SHAPES = 4;
 constexpr uint32_t TILE_SHAPE_W[N_TILE_SHAPES] = {16, 32, 64, 8};
 constexpr uint32_t TILE_SHAPE_H[N_TILE_SHAPES] = {16, 8, 4, 32};
 
+// Flow toward the other keyframe (full_ff): x, y, z only. Its fourth
+// component (the visibility flag, also in cam_ff.w) is never read, so the
+// plane stores 12 bytes per pixel instead of 16.
+struct Flow3 {
+    float x, y, z;
+};
+
 // One source view of the reproject+splat stage (K1, renderer.cu:76-146).
 struct SplatJob {
     const float *z;
     const float *mx, *my, *mz;  // null: no object motion
-    float4 *full_ff;
+    Flow3 *full_ff;
     float4 *cam_ff;
     unsigned long long *splat;
     uint32_t w, h, n_px, block_begin;
@@ -60,7 +67,7 @@ struct ResolveView {
     const uchar4 *sh_a;
     const uchar4 *sh_b;
     const float4 *cam_ff;
-    const float4 *full_ff;
+    const Flow3 *full_ff;
     const unsigned long long *splat;
     uint32_t w, h;
     int32_t priority;

@@ -234,7 +234,7 @@ __device__ __forceinline__ uchar4 second_chance(float weight, const FrameConst &
                                                 const ResolveView &from, const ResolveView &to, float eps2) {
     const bool sh_from = uses_shadows(fc.sidx, fc.sfpa, fc.no_shadows, from);
     const bool sh_to = uses_shadows(fc.sidx, fc.sfpa, fc.no_shadows, to);
-    const float4 m_ft = fetch(from.full_ff, from.w, from.h, si.x, si.y);
+    const Flow3 m_ft = fetch(from.full_ff, from.w, from.h, si.x, si.y);
     const float sz = fetch(from.z, from.w, from.h, si.x, si.y);
     const Texels tq = load_texels(from, si.x, si.y, sh_from);
     const float proj_x = cs_coord(si.x, from.inv_w) + m_ft.x, proj_y = cs_coord(si.y, from.inv_h) + m_ft.y;
@@ -246,7 +246,7 @@ __device__ __forceinline__ uchar4 second_chance(float weight, const FrameConst &
     if (in_range(pp, to.w, to.h)) {
         const uint32_t ux = (uint32_t)pp.x, uy = (uint32_t)pp.y;
         const float zt = at(to.z, to.w, ux, uy);
-        const float4 m_tf = at(to.full_ff, to.w, ux, uy);
+        const Flow3 m_tf = at(to.full_ff, to.w, ux, uy);
         const Texels tp = load_texels(to, ux, uy, sh_to);
         const float dx = m_ft.x - (-m_tf.x), dy = m_ft.y - (-m_tf.y), dz = m_ft.z - (-m_tf.z);
         const float dist = sqrtf(dx * dx + dy * dy + dz * dz);

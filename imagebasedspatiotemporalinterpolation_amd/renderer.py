@@ -249,7 +249,7 @@ class Renderer:
         return {
             "splat": N.device_to_host(sv.splat, n * 8, np.uint64, (sv.height, sv.width)),
             "cam_ff": N.device_to_host(sv.cam_ff, n * 16, np.float32, (sv.height, sv.width, 4)),
-            "full_ff": N.device_to_host(sv.full_ff, n * 16, np.float32, (sv.height, sv.width, 4)),
+            "full_ff": N.device_to_host(sv.full_ff, n * 12, np.float32, (sv.height, sv.width, 3)),
         }
 
 

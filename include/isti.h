@@ -131,9 +131,11 @@ typedef struct isti_config {
 typedef struct isti_renderer isti_renderer;
 
 /* Device scratch of one (slot, view) after the last render, for inspection:
- * the u64 splat buffer and the two float4 flow planes (cam_ff: source ->
- * target camera, full_ff: source -> other keyframe), all device pointers,
- * rows packed. Reference: Renderer members, include/renderer.h:64-92. */
+ * the u64 splat buffer and the two flow planes, all device pointers, rows
+ * packed: cam_ff (source -> target camera) holds 4 floats per pixel (x, y, z,
+ * visible-in-main flag), full_ff (source -> other keyframe) 3 floats (x, y,
+ * z; the reference's fourth component repeats the flag and is never read).
+ * Reference: Renderer members, include/renderer.h:64-92. */
 typedef struct isti_scratch_view {
     uint32_t width;
     uint32_t height;
