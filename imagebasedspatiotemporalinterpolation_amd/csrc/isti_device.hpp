@@ -85,6 +85,10 @@ __device__ __forceinline__ int2 cs2pixel(float cx, float cy, uint32_t w, uint32_
     r.y = cvt_s32(floorf((cy + 1.0f) * (float)h * 0.5f));
     return r;
 }
+// in_range for coordinates that may be negative: one unsigned compare per axis.
+__device__ __forceinline__ bool in_range_u(int2 p, uint32_t w, uint32_t h) {
+    return (uint32_t)p.x < w && (uint32_t)p.y < h;
+}
 __device__ __forceinline__ float4 mat_vec(const Mat4 &M, float4 v) {
     const float *m = M.m;
     float4 r;

@@ -112,7 +112,7 @@ __device__ __forceinline__ void chain_start(Chain &c, bool run, uint32_t x, uint
 __device__ __forceinline__ void chain_step(Chain &c, int i, int n_iter, float2 pcs_full, const ResolveView &v) {
     if (!c.active) return;
     const int2 t = cs2pixel(pcs_full.x - c.m.x, pcs_full.y - c.m.y, v.w, v.h);
-    if (!in_range(t, v.w, v.h)) {
+    if (!in_range_u(t, v.w, v.h)) {
         c.active = false;
         return;
     }
