@@ -99,10 +99,9 @@ struct Chain {
 constexpr uint32_t PX_NONE = 0xffffffffu;  // x, y <= 65534: never a packed pixel
 
 // A chain that does not run (no splat candidate, or the skipped fwd search)
-// parks at pixel (0, 0), so the unconditional reloads in chain_step stay
-// inside the view; its result is never used.
+// stays inactive at its window pixel (which the always-valid blend reads).
 __device__ __forceinline__ void chain_start(Chain &c, bool run, uint32_t x, uint32_t y, const ResolveView &v) {
-    c.h0 = run ? pack_px(x, y) : 0u;
+    c.h0 = pack_px(x, y);  // a chain that does not run keeps its window pixel
     c.h1 = c.h2 = c.h3 = PX_NONE;
     c.active = run;
     c.m = run ? at_xy(v.cam_ff, v.w, x, y) : make_float2(0.f, 0.f);
@@ -498,11 +497,12 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
             const ResolveView &v = d ? a.bwd[c] : a.fwd[c];
             err[c][d] = FLT_MAX;
             vis[c][d] = false;
+            // the search result, or the window pixel of a search that did not
+            // run (its depth is never read: its error is FLT_MAX)
+            const uint32_t qx = ch[c][d].h0 & 0xFFFFu, qy = ch[c][d].h0 >> 16;
+            si[c][d] = SplatInfo{0.0f, qx, qy};
             if (ran[c][d]) {
-                const uint32_t qx = ch[c][d].h0 & 0xFFFFu, qy = ch[c][d].h0 >> 16;
                 const float4 m = at(v.cam_ff, v.w, qx, qy);
-                si[c][d].x = qx;
-                si[c][d].y = qy;
                 si[c][d].depth = z_to_depth(at(v.z, v.w, qx, qy) + m.z);
                 err[c][d] = reproj_error(make_uint2(qx, qy), v, cpcs[c], m);
                 vis[c][d] = m.w > 0.0f;

@@ -7,11 +7,11 @@
 namespace isti {
 
 // Occupancy floor per variant (waves per SIMD). The pass is bound by the
-// latency of its dependent gathers, so waves in flight beat a few spilled
-// registers: the 2-aux window-1 variant (1080p + 2 aux workload) runs 11 %
-// faster at 7 waves (72 VGPRs, 16 B/lane spilled) than at its natural 5, and
-// slower again at 8 (48 B/lane spilled). 1 = no floor.
-constexpr int resolve_min_waves(int w, int naux) { return w != 1 ? 1 : (naux == 2 ? 7 : (naux == 3 ? 5 : 1)); }
+// latency of its dependent gathers, so waves in flight pay: the 2-aux
+// window-1 variant (1080p + 2 aux workload) fits 64 VGPRs without spilling
+// and runs at 8 waves, the 3-aux one at 6 (79 VGPRs). A floor that forces
+// spills costs more than the waves gain. 1 = no floor.
+constexpr int resolve_min_waves(int w, int naux) { return w != 1 ? 1 : (naux == 2 ? 8 : (naux == 3 ? 6 : 1)); }
 
 template <int W, bool MAV, int NAUX, int TW, int TH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(resolve_min_waves(W, NAUX))))
