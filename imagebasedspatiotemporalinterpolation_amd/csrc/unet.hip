@@ -160,6 +160,97 @@ __global__ __launch_bounds__(256) void conv_kernel(Src s, const uint16_t *__rest
     }
 }
 
+// 3x3 convolution with the input staged in LDS: a block computes 4 rows x
+// 16*MT pixels x NT*16 outputs (one row per wave) and walks the input
+// channels in chunks of 16. Each chunk's 6-row halo is loaded once into LDS
+// (zero outside the image), then the nine taps read their A fragments from
+// LDS, so a chunk costs one round trip to memory instead of nine. The K
+// order is chunk-major (c0, then ky, kx), so sums are rounded in a different
+// order than conv_kernel's.
+template <int NT, int MT>
+__global__ __launch_bounds__(256) void conv3_lds_kernel(Src s, const uint16_t *__restrict__ w,
+                                                        const float *__restrict__ bias, int cin, int N, int H, int W,
+                                                        int cout, uint16_t *out_bf, float *out_f, int relu) {
+    constexpr int PXW = 16 * MT, COLS = PXW + 2, QUADS = 4;  // 4 channels per 8-byte quad
+    __shared__ uint2 tile[6][COLS][QUADS];
+    const int lane = threadIdx.x & 63, wv = (int)(threadIdx.x >> 6);
+    const int chunks = (W + PXW - 1) / PXW, rgroups = (H + 3) / 4;
+    const int b = (int)blockIdx.x;
+    const int n = b / (rgroups * chunks);
+    const int rem = b - n * rgroups * chunks;
+    const int ry = rem / chunks, x0 = (rem - ry * chunks) * PXW;
+    const int y0 = ry * 4, y = y0 + wv;
+    const int kq = (lane >> 4) * 4;
+    const int co_base = (int)blockIdx.y * NT * 16;
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; m++)
+#pragma unroll
+        for (int t = 0; t < NT; t++) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int c0 = 0; c0 < cin; c0 += 16) {
+        for (int i = (int)threadIdx.x; i < 6 * COLS * QUADS; i += 256) {
+            const int r = i / (COLS * QUADS), rc = i - r * COLS * QUADS;
+            const int col = rc / QUADS, q = rc - col * QUADS;
+            const int yy = y0 - 1 + r, xx = x0 - 1 + col;
+            bf16x4 v = bf16x4{0, 0, 0, 0};
+            if (yy >= 0 && yy < H && xx >= 0 && xx < W) v = load_src4(s, n, yy, xx, H, W, c0 + q * 4);
+            uint2 u;
+            u.x = (uint32_t)(uint16_t)v[0] | ((uint32_t)(uint16_t)v[1] << 16);
+            u.y = (uint32_t)(uint16_t)v[2] | ((uint32_t)(uint16_t)v[3] << 16);
+            tile[r][col][q] = u;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ky = 0; ky < 3; ky++) {
+#pragma unroll
+            for (int kx = 0; kx < 3; kx++) {
+                bf16x4 av[MT];
+#pragma unroll
+                for (int m = 0; m < MT; m++) {
+                    const uint2 u = tile[wv + ky][m * 16 + (lane & 15) + kx][kq >> 2];
+                    av[m][0] = (short)(u.x & 0xffffu);
+                    av[m][1] = (short)(u.x >> 16);
+                    av[m][2] = (short)(u.y & 0xffffu);
+                    av[m][3] = (short)(u.y >> 16);
+                }
+#pragma unroll
+                for (int t = 0; t < NT; t++) {
+                    const uint2 bv2 = *reinterpret_cast<const uint2 *>(
+                        w + (size_t)((co_base + t * 16 + (lane & 15)) * 9 + ky * 3 + kx) * cin + c0 + kq);
+                    bf16x4 bv;
+                    bv[0] = (short)(bv2.x & 0xffffu);
+                    bv[1] = (short)(bv2.x >> 16);
+                    bv[2] = (short)(bv2.y & 0xffffu);
+                    bv[3] = (short)(bv2.y >> 16);
+#pragma unroll
+                    for (int m = 0; m < MT; m++)
+                        acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[m], bv, acc[m][t], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();  // the next chunk overwrites the tile
+    }
+    if (y >= H) return;
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        const int co = co_base + t * 16 + (lane & 15);
+        const float bsv = bias ? bias[co] : 0.0f;
+#pragma unroll
+        for (int m = 0; m < MT; m++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int x = x0 + m * 16 + kq + i;
+                if (x >= W) continue;
+                float v = acc[m][t][i] + bsv;
+                if (relu) v = v > 0.0f ? v : 0.0f;
+                const size_t o = (((size_t)n * H + y) * W + x) * cout + co;
+                if (out_f) out_f[o] = v;
+                else out_bf[o] = f2bf(v);
+            }
+        }
+    }
+}
+
 // dw[co][kk] += sum_p g[p][co] * in[p + off(kk)][ci(kk)], kk = (ky*ks+kx)*cin+ci
 __global__ __launch_bounds__(256) void wgrad_kernel(Src s, const uint16_t *__restrict__ g, int cout, int cin, int ks,
                                                     int N, int H, int W, float *dw) {
@@ -359,6 +450,13 @@ unsigned blocks_for(long n, K) {
 template <int NT, int MT>
 hipError_t launch_conv_tiles(const Src &s, const uint16_t *w, const float *bias, int cin, int cout, int ks, int N, int H,
                              int W, uint16_t *out_bf, float *out_f, int relu, hipStream_t st) {
+    if (ks == 3) {  // LDS-staged input, one block per 4 rows x 16*MT pixels
+        const long blocks = (long)N * ((H + 3) / 4) * ((W + 16 * MT - 1) / (16 * MT));
+        const dim3 grid((unsigned)blocks, (unsigned)(cout / (16 * NT))), block(256);
+        hipLaunchKernelGGL((conv3_lds_kernel<NT, MT>), grid, block, 0, st, s, w, bias, cin, N, H, W, cout, out_bf, out_f,
+                           relu);
+        return hipGetLastError();
+    }
     const long waves = (long)N * H * ((W + 16 * MT - 1) / (16 * MT));
     const dim3 grid((unsigned)((waves + 3) / 4), (unsigned)(cout / (16 * NT))), block(256);
     hipLaunchKernelGGL((conv_kernel<NT, MT>), grid, block, 0, st, s, w, bias, cin, ks, N, H, W, cout, out_bf, out_f,
