@@ -163,10 +163,11 @@ __global__ __launch_bounds__(256) void conv_kernel(Src s, const uint16_t *__rest
 // 3x3 convolution with the input staged in LDS: a block computes 4 rows x
 // 16*MT pixels x NT*16 outputs (one row per wave) and walks the input
 // channels in chunks of 16. Each chunk's 6-row halo is loaded once into LDS
-// (zero outside the image), then the nine taps read their A fragments from
-// LDS, so a chunk costs one round trip to memory instead of nine. The K
-// order is chunk-major (c0, then ky, kx), so sums are rounded in a different
-// order than conv_kernel's.
+// (zero outside the image) - the next chunk's into registers while the
+// current one is multiplied - and the nine taps read their A fragments from
+// LDS, so a chunk costs one round trip to memory instead of nine, overlapped
+// with the previous chunk's MFMAs. The K order is chunk-major (c0, then ky,
+// kx), so sums are rounded in a different order than conv_kernel's.
 template <int NT, int MT>
 __global__ __launch_bounds__(256) void conv3_lds_kernel(Src s, const uint16_t *__restrict__ w,
                                                         const float *__restrict__ bias, int cin, int N, int H, int W,
@@ -187,19 +188,38 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(Src s, const uint16_t *_
     for (int m = 0; m < MT; m++)
 #pragma unroll
         for (int t = 0; t < NT; t++) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int c0 = 0; c0 < cin; c0 += 16) {
-        for (int i = (int)threadIdx.x; i < 6 * COLS * QUADS; i += 256) {
-            const int r = i / (COLS * QUADS), rc = i - r * COLS * QUADS;
-            const int col = rc / QUADS, q = rc - col * QUADS;
-            const int yy = y0 - 1 + r, xx = x0 - 1 + col;
-            bf16x4 v = bf16x4{0, 0, 0, 0};
-            if (yy >= 0 && yy < H && xx >= 0 && xx < W) v = load_src4(s, n, yy, xx, H, W, c0 + q * 4);
-            uint2 u;
-            u.x = (uint32_t)(uint16_t)v[0] | ((uint32_t)(uint16_t)v[1] << 16);
-            u.y = (uint32_t)(uint16_t)v[2] | ((uint32_t)(uint16_t)v[3] << 16);
-            tile[r][col][q] = u;
+    constexpr int ITEMS = 6 * COLS * QUADS, PER = (ITEMS + 255) / 256;
+    uint2 pre[PER];
+    auto load_chunk = [&](int c) {
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = (int)threadIdx.x + k * 256;
+            pre[k] = make_uint2(0u, 0u);
+            if (i < ITEMS) {
+                const int r = i / (COLS * QUADS), rc = i - r * COLS * QUADS;
+                const int col = rc / QUADS, q = rc - col * QUADS;
+                const int yy = y0 - 1 + r, xx = x0 - 1 + col;
+                if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+                    const bf16x4 v = load_src4(s, n, yy, xx, H, W, c + q * 4);
+                    pre[k].x = (uint32_t)(uint16_t)v[0] | ((uint32_t)(uint16_t)v[1] << 16);
+                    pre[k].y = (uint32_t)(uint16_t)v[2] | ((uint32_t)(uint16_t)v[3] << 16);
+                }
+            }
         }
-        __syncthreads();
+    };
+    auto store_chunk = [&]() {
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = (int)threadIdx.x + k * 256;
+            if (i < ITEMS) (&tile[0][0][0])[i] = pre[k];
+        }
+    };
+    load_chunk(0);
+    store_chunk();
+    __syncthreads();
+    for (int c0 = 0; c0 < cin; c0 += 16) {
+        const bool more = c0 + 16 < cin;
+        if (more) load_chunk(c0 + 16);  // in flight during this chunk's MFMAs
 #pragma unroll
         for (int ky = 0; ky < 3; ky++) {
 #pragma unroll
@@ -229,6 +249,10 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(Src s, const uint16_t *_
             }
         }
         __syncthreads();  // the next chunk overwrites the tile
+        if (more) {
+            store_chunk();
+            __syncthreads();
+        }
     }
     if (y >= H) return;
 #pragma unroll
@@ -468,7 +492,7 @@ hipError_t launch_conv_tiles(const Src &s, const uint16_t *w, const float *bias,
 // blocking that still gives the launch CONV_MIN_WAVES waves (2 per SIMD of
 // the chip), else the most waves. Deep levels are small (60 x 34 pixels),
 // so their output channels are split over the grid's y dimension.
-constexpr long CONV_MIN_WAVES = 2048;
+constexpr long CONV_MIN_WAVES = 8192;
 
 hipError_t launch_conv(const Src &s, const uint16_t *w, const float *bias, int cin, int cout, int ks, int N, int H,
                        int W, uint16_t *out_bf, float *out_f, int relu, hipStream_t st) {
