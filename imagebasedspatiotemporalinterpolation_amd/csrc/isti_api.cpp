@@ -321,6 +321,7 @@ isti_status plan_frame(isti_renderer *r, const isti_render_info *info, uint8_t *
             J.weight = sl == 0 ? info->alpha : 1.0f - info->alpha;
             J.main_slot = sl;
             J.inv_w = 1.0 / (double)d.width;
+            J.rcp_w = (float)(1.0 / (double)d.width);
             J.inv_h = 1.0 / (double)d.height;
             J.inv_vp_from = to_mat(d.inv_viewproj);
             J.vp_to = to_mat(other.viewproj);

@@ -182,6 +182,18 @@ __device__ __forceinline__ uint32_t key_y(unsigned long long k) { return (uint32
 // a 32-bit multiply would take four passes.
 __device__ __forceinline__ uint32_t px_index(uint32_t w, uint32_t x, uint32_t y) { return __umul24(y, w) + x; }
 
+// Row of linear pixel p < w h (w, h <= 65535) without an integer divide:
+// RN32(RN32(p) RN32(1/w)) errs by less than 3 2^-24 p / w < 3 2^-8 (p / w <
+// h < 2^16), so its truncation is the row or one off, and one remainder
+// check corrects it.
+__device__ __forceinline__ uint32_t div_px(uint32_t p, uint32_t w, float rcp_w) {
+    uint32_t q = (uint32_t)((float)p * rcp_w);
+    const int32_t r = (int32_t)(p - __umul24(q, w));
+    if (r < 0) q -= 1u;
+    else if (r >= (int32_t)w) q += 1u;
+    return q;
+}
+
 // Element idx of a plane through a 32-bit byte offset: views hold fewer than
 // 2^28 pixels (ISTI_MAX_VIEW_PIXELS), so even float4 planes stay below 4 GiB,
 // and the load takes the scalar base + 32-bit vector offset form (no 64-bit

@@ -101,8 +101,8 @@ template <typename J_t, typename M_t>
 __device__ __forceinline__ void splat_pixel(const J_t &J, const M_t &vp_cam, const M_t &vp_main, bool init_linear,
                                             uint32_t p) {
     if (p >= J.n_px) return;
-    const uint32_t y = p / J.w;
-    const uint32_t x = p - y * J.w;
+    const uint32_t y = div_px(p, J.w, J.rcp_w);
+    const uint32_t x = p - __umul24(y, J.w);
     uint32_t tx, ty;
     unsigned long long key;
     if (project_pixel(J, vp_cam, vp_main, init_linear, x, y, p, tx, ty, key))

@@ -127,3 +127,27 @@ def test_shadow_code_min_max_bit_parallel():
                 inv = {1: 1.0, 2: 0.5, 3: 1.0 / 3.0}.get(b, np.nan)
                 fast = np.float32(np.float64(a) * inv)
                 assert ref == fast or (np.isnan(ref) and np.isnan(fast)) or (b == 0 and a > 0), (a, b)
+
+
+def test_pixel_row_from_float_reciprocal():
+    """K1's row of a linear pixel, p // w for p < w h (w, h <= 65535), as
+    trunc(RN32(RN32(p) * RN32(1/w))) with one remainder correction:
+    exhaustive over p for small and awkward widths, sampled for the rest."""
+    f32 = np.float32
+
+    def fast(p, w):
+        rcp = f32(1.0 / w)
+        q = (p.astype(f32) * rcp).astype(np.float64).astype(np.int64)  # v_cvt_u32_f32 truncates
+        r = p - q * w
+        return np.where(r < 0, q - 1, np.where(r >= w, q + 1, q))
+
+    rng = np.random.default_rng(9)
+    widths = list(range(1, 65)) + [97, 127, 255, 511, 640, 960, 1023, 1280, 1920, 2064, 4095, 4096, 8191, 65535] \
+        + [int(v) for v in rng.integers(65, 65536, 30)]
+    for w in widths:
+        h = min(65535, (2**28 - 1) // w)
+        n = w * h
+        p = np.arange(n, dtype=np.int64) if n <= 1_000_000 else np.unique(np.concatenate([
+            rng.integers(0, n, 300_000), np.arange(w) * 0 + n - 1 - np.arange(w), (np.arange(1, h) * w)[-5000:],
+            (np.arange(1, h) * w)[-5000:] - 1]))
+        assert np.array_equal(fast(p, w), p // w), w
