@@ -138,6 +138,17 @@ __device__ __forceinline__ float4 project(const Mat4 &M, float4 v) {
     const float3 q = div3_rn(r.x, r.y, r.z, w);
     return make_float4(q.x, q.y, q.z, ww);
 }
+// project() for a point that further mat_vec calls consume: w is returned as
+// the literal 1 (so their m[12+i] * w products fold away) and a bad w (not
+// finite, or zero) is carried as NaN in x instead. Every row of a later
+// mat_vec holds an m * x term, so it is NaN exactly where project()'s NaN w
+// would have made it NaN.
+__device__ __forceinline__ float4 project_unit(const Mat4 &M, float4 v) {
+    const float4 r = mat_vec(M, v);
+    const float w = r.w;
+    const float3 q = div3_rn(r.x, r.y, r.z, w);
+    return make_float4((__builtin_isfinite(w) && w != 0.0f) ? q.x : __builtin_nanf(""), q.y, q.z, 1.0f);
+}
 // helper_common.h:89-95
 __device__ __forceinline__ float z_to_depth(float z) { return (z + 1.0f) * 0.5f; }
 __device__ __forceinline__ float depth_to_z(float d) { return d * 2.0f - 1.0f; }

@@ -60,9 +60,9 @@ __device__ __forceinline__ bool project_pixel(const J_t &J, const M_t &vp_cam, c
     if (J.mx != nullptr) om = make_float4(J.mx[p], J.my[p], J.mz[p], 0.f);
 
     const float4 cs_from = make_float4(cs_coord(x, J.inv_w), cs_coord(y, J.inv_h), z_from, 1.0f);
-    const float4 ws_from = project(J.inv_vp_from, cs_from);
+    const float4 ws_from = project_unit(J.inv_vp_from, cs_from);
     const float4 cs_moved = make_float4(cs_from.x + om.x, cs_from.y + om.y, cs_from.z + om.z, cs_from.w + 0.0f);
-    const float4 ws_to = project(J.inv_vp_from, cs_moved);
+    const float4 ws_to = project_unit(J.inv_vp_from, cs_moved);
 
     const float4 cs_main = project(vp_main, ws_from);
     const float lim = 1.0f - 1e-3f;
@@ -70,9 +70,13 @@ __device__ __forceinline__ bool project_pixel(const J_t &J, const M_t &vp_cam, c
                          cs_main.z > -lim && cs_main.z < lim;
     const float vis = visible ? 1.0f : 0.0f;
 
+    // lerp of the two world points; both w are 1, so the lerped w is
+    // 1 + wt * 0: 1 for a finite weight and NaN (carried in x) otherwise
     const float wt = J.weight;
-    const float4 ws_scaled = make_float4(ws_from.x + wt * (ws_to.x - ws_from.x), ws_from.y + wt * (ws_to.y - ws_from.y),
-                                         ws_from.z + wt * (ws_to.z - ws_from.z), ws_from.w + wt * (ws_to.w - ws_from.w));
+    const float sx = ws_from.x + wt * (ws_to.x - ws_from.x);
+    const float4 ws_scaled = make_float4(__builtin_isfinite(wt) ? sx : __builtin_nanf(""),
+                                         ws_from.y + wt * (ws_to.y - ws_from.y),
+                                         ws_from.z + wt * (ws_to.z - ws_from.z), 1.0f);
     const float4 cs_to = project(J.vp_to, ws_to);
     const float fx = cs_to.x - cs_from.x, fy = cs_to.y - cs_from.y, fz = cs_to.z - cs_from.z;
     J.full_ff[p] = Flow3{fx, fy, fz};

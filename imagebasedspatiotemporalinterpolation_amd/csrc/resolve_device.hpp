@@ -120,8 +120,9 @@ __device__ __forceinline__ void chain_step(Chain &c, int i, int n_iter, float2 p
         c.active = false;
         return;
     }
-    const int p = nt == c.h1 ? 2 : (nt == c.h2 ? 3 : (nt == c.h3 ? 4 : 0));
-    if (p != 0) {
+    // three compares whose masks are OR-ed, without a short-circuit branch
+    if ((int)(nt == c.h1) | (int)(nt == c.h2) | (int)(nt == c.h3)) {
+        const int p = nt == c.h1 ? 2 : (nt == c.h2 ? 3 : 4);
         const int idx = p - 1 - (n_iter - (i + 1)) % p;  // c_r = h_{p-1-r}
         const uint32_t q = idx == 0 ? c.h0 : (idx == 1 ? c.h1 : (idx == 2 ? c.h2 : c.h3));
         c.h0 = q;  // the chain stops: m is not needed again
@@ -357,12 +358,14 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
         constexpr int PER = (NAUX * 2 * TW * TH + (int)RESOLVE_THREADS - 1) / (int)RESOLVE_THREADS;
         unsigned long long res[PER];
         int nx[NC], cnt[NC], first[NC + 1];
+        float rcp_nx[NC];  // RN32(1/nx): cell rows by div_px instead of an integer divide
         first[1] = 0;
 #pragma unroll
         for (int c = 1; c < NC; c++) {
             nx[c] = span_x[c] - 2 * W;
             cnt[c] = nx[c] * (span_y[c] - 2 * W);
             first[c + 1] = first[c] + 2 * cnt[c];
+            rcp_nx[c] = 1.0f / (float)nx[c];
         }
 #pragma unroll
         for (int r = 0; r < PER; r++) {
@@ -372,7 +375,7 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
             for (int c = 1; c < NC; c++) {
                 if (k >= first[c] && k < first[c + 1]) {
                     const int j = k - first[c], d = j >= cnt[c] ? 1 : 0, p = j - d * cnt[c];
-                    const int py = p / nx[c], px = p - py * nx[c];
+                    const int py = (int)div_px((uint32_t)p, (uint32_t)nx[c], rcp_nx[c]), px = p - py * nx[c];
                     const SplatInfo w = init_search_lds<W>(s_splat[c][d], SPAN, px + W, py + W);
                     res[r] = encode_key(w.depth, w.x, w.y);
                 }
