@@ -179,9 +179,13 @@ __device__ __forceinline__ uchar4 lerp_to_u8(uchar4 a, uchar4 b, float t) {
     return make_uchar4(lerp_ch(a.x, b.x, t), lerp_ch(a.y, b.y, t), lerp_ch(a.z, b.z, t), lerp_ch(a.w, b.w, t));
 }
 
-// renderer.cu:59-74 splat key: depth bits << 32 | x << 16 | y
+// renderer.cu:59-74 splat key: depth bits << 32 | x << 16 | y. A NaN depth
+// is encoded as 0x7fffffff, the canonical NaN that NVIDIA GPUs return from
+// binary32 arithmetic, so a NaN key sorts after every number and before the
+// empty-cell value whatever sign and payload the NaN had here.
 __device__ __forceinline__ unsigned long long encode_key(float depth, uint32_t x, uint32_t y) {
-    return ((unsigned long long)__float_as_uint(depth) << 32) | (unsigned long long)((x << 16) | y);
+    const uint32_t bits = __builtin_isnan(depth) ? 0x7fffffffu : __float_as_uint(depth);
+    return ((unsigned long long)bits << 32) | (unsigned long long)((x << 16) | y);
 }
 __device__ __forceinline__ float key_depth(unsigned long long k) { return __uint_as_float((uint32_t)(k >> 32)); }
 __device__ __forceinline__ uint32_t key_x(unsigned long long k) { return (uint32_t)k >> 16; }

@@ -124,10 +124,13 @@ inline C4 lerp_to_u8(C4 a, C4 b, float t) {
     return C4{ch(a.x, b.x), ch(a.y, b.y), ch(a.z, b.z), ch(a.w, b.w)};
 }
 
-// renderer.cu:59-74 splat key codec: depth bits << 32 | x << 16 | y
+// renderer.cu:59-74 splat key codec: depth bits << 32 | x << 16 | y. The
+// reference runs on NVIDIA GPUs, whose binary32 arithmetic returns the
+// canonical NaN 0x7fffffff; a NaN depth is encoded with those bits (an x86
+// NaN is negative and would sort after the empty-cell value).
 inline uint64_t encode_key(float depth, U2 src) {
-    uint32_t bits;
-    std::memcpy(&bits, &depth, 4);
+    uint32_t bits = 0x7fffffffu;
+    if (depth == depth) std::memcpy(&bits, &depth, 4);
     return ((uint64_t)bits << 32) | (uint64_t)((src.x << 16) | src.y);
 }
 struct SplatInfo { float depth; U2 src; };

@@ -133,6 +133,39 @@ def test_c2_full_1080p(oracle_lib):
     _check(oracle_lib, cfg, fwd, bwd, 4, max_iter=32)
 
 
+def test_degenerate_unprojection_nan_column(oracle_lib, c1):
+    """The fwd keyframe's unprojection has w == 0 on one pixel column (the
+    projective division w / w of CS2WS, helper_common.h:77-87, is then NaN):
+    its flows are NaN on both sides and every other value matches. NaN depths
+    reach the splat as keys with the canonical NaN 0x7fffffff (the reference's
+    GPUs return it), which win cells no number reaches: the splat buffers
+    are compared byte for byte; flows match NaN for NaN."""
+    cfg, (fwd, bwd) = c1
+    w = cfg.width
+    x0 = w // 3
+    cx0 = np.float32(2.0) * (np.float32(x0) + np.float32(0.5)) / np.float32(w) - np.float32(1.0)
+    ivp = fwd[0]["inv_viewproj"].copy()
+    ivp[3], ivp[7], ivp[11], ivp[15] = 1.0, 0.0, 0.0, -cx0  # w row: cs.x - cs.x(x0)
+    fwd2 = [dict(fwd[0], inv_viewproj=ivp)] + list(fwd[1:])
+    fpa = cfg.keyframe_spacing
+    vp, tivp, tw, th = F.target(cfg, 0, 2)
+    prev = _prev(th, tw)
+    print(f"case: {tw}x{th} degenerate unprojection column {x0}", flush=True)
+    ref, sref = F.oracle_frame(oracle_lib, fwd2, bwd, vp, tivp, 2 / fpa, fpa, 0, prev=prev)
+    got, sgot = F.gpu_frame(fwd2, bwd, vp, tivp, 2 / fpa, fpa, 0, prev=prev)
+    cf = sgot[(0, 0)]["cam_ff"]  # full_ff is NaN there only where the motion is zero
+    assert np.isnan(cf[:, x0, 0]).all() and not np.isnan(cf[:, x0 + 1, 0]).any()
+    nan_keys = (sgot[(0, 0)]["splat"] >> np.uint64(32)) == np.uint64(0x7FFFFFFF)
+    assert nan_keys.any()  # the case reaches the NaN-key path
+    for key, s in sref.items():
+        g = sgot[key]
+        np.testing.assert_array_equal(g["splat"], s["splat"], err_msg=f"splat {key}")
+        assert F.flow_err_px(g["cam_ff"], s["cam_ff"]) <= 1e-3, f"cam_ff {key}"
+        assert F.flow_err_px(g["full_ff"], s["full_ff"]) <= 1e-3, f"full_ff {key}"
+        np.testing.assert_array_equal(g["cam_ff"][..., 3], s["cam_ff"][..., 3])
+    F.assert_rgb_parity(got, ref)
+
+
 def test_repeat_run_bit_identical(c1):
     """K1's atomicMin is order-independent: two renders give identical bytes."""
     cfg, (fwd, bwd) = c1

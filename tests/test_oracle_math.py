@@ -97,6 +97,13 @@ def test_key_codec_roundtrip_and_order(oracle_lib):
         depths.append(np.float32(depth))
     order_k = np.argsort(np.array(keys, dtype=np.uint64), kind="stable")
     assert np.all(np.diff(np.array(depths)[order_k]) >= 0)
+    # a NaN depth (either sign) takes the canonical NaN 0x7fffffff of the
+    # reference's GPUs: after every finite or infinite depth, before the clear value
+    for nan_bits in (0x7FC00000, 0xFFC00000, 0x7F800001):
+        nan = float(np.array([nan_bits], np.uint32).view(np.float32)[0])
+        k = L.oracle_encode_key(nan, 5, 7)
+        assert k == (0x7FFFFFFF << 32) | (5 << 16) | 7
+        assert L.oracle_encode_key(float("inf"), 0xFFFF, 0xFFFF) < k < 0xFEFEFEFEFEFEFEFE
     # the 0xfe.. clear value decodes to a negative depth (an empty cell)
     L.oracle_decode_key(0xFEFEFEFEFEFEFEFE, ctypes.byref(d), ctypes.byref(x), ctypes.byref(y))
     assert d.value < 0 and math.isfinite(d.value) and x.value == 0xFEFE and y.value == 0xFEFE
