@@ -101,6 +101,10 @@ __device__ __forceinline__ float4 mat_vec(const Mat4 &M, float4 v) {
 // Class masks of v_cmp_class_f32 (bit 2 -inf, 3 -normal, 8 +normal, 9 +inf).
 constexpr int CLASS_NORMAL = (1 << 3) | (1 << 8);
 constexpr int CLASS_NORMAL_OR_INF = CLASS_NORMAL | (1 << 2) | (1 << 9);
+constexpr int CLASS_FINITE_NONZERO = CLASS_NORMAL | (1 << 4) | (1 << 7);  // + -subnormal
+// Midpoint between RN32(1 - RN32(1e-3)) = 16760439 * 2^-24 (the main-view
+// visibility limit of K1, renderer.cu:76-146) and its predecessor.
+constexpr double LIM_MID = 33520877.0 / 33554432.0;
 
 // RN32(n / w) for the three numerators of a projective division, equal to
 // the IEEE quotients. For a normal w, R = 1/w comes from the hardware

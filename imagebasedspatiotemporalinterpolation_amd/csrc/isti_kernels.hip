@@ -64,10 +64,24 @@ __device__ __forceinline__ bool project_pixel(const J_t &J, const M_t &vp_cam, c
     const float4 cs_moved = make_float4(cs_from.x + om.x, cs_from.y + om.y, cs_from.z + om.z, cs_from.w + 0.0f);
     const float4 ws_to = project_unit(J.inv_vp_from, cs_moved);
 
-    const float4 cs_main = project(vp_main, ws_from);
-    const float lim = 1.0f - 1e-3f;
-    const bool visible = cs_main.x > -lim && cs_main.x < lim && cs_main.y > -lim && cs_main.y < lim &&
-                         cs_main.z > -lim && cs_main.z < lim;
+    // visible in the main view: -lim < RN32(c / w) < lim for c = x, y, z, with
+    // lim = RN32(1 - RN32(1e-3)) = 16760439 * 2^-24. RN32 is monotone and lim's
+    // significand is odd, so RN32(q) < lim exactly when q <= LIM_MID, the
+    // midpoint below lim (a tie rounds to the even neighbour below). For w
+    // finite and non-zero that is |c| <= LIM_MID |w|, and LIM_MID (25 bits)
+    // times w (24 bits) is exact in binary64: no division is needed. A zero,
+    // infinite or NaN w takes the IEEE quotients.
+    const float4 rm = mat_vec(vp_main, ws_from);
+    bool visible;
+    if (__builtin_amdgcn_classf(rm.w, CLASS_FINITE_NONZERO)) {
+        const double bound = LIM_MID * __builtin_fabs((double)rm.w);
+        visible = __builtin_fabs((double)rm.x) <= bound && __builtin_fabs((double)rm.y) <= bound &&
+                  __builtin_fabs((double)rm.z) <= bound;
+    } else {
+        const float lim = 1.0f - 1e-3f;
+        const float qx = rm.x / rm.w, qy = rm.y / rm.w, qz = rm.z / rm.w;
+        visible = qx > -lim && qx < lim && qy > -lim && qy < lim && qz > -lim && qz < lim;
+    }
     const float vis = visible ? 1.0f : 0.0f;
 
     // lerp of the two world points; both w are 1, so the lerped w is

@@ -151,3 +151,44 @@ def test_pixel_row_from_float_reciprocal():
             rng.integers(0, n, 300_000), np.arange(w) * 0 + n - 1 - np.arange(w), (np.arange(1, h) * w)[-5000:],
             (np.arange(1, h) * w)[-5000:] - 1]))
         assert np.array_equal(fast(p, w), p // w), w
+
+
+def _visible_fast(c, w, bound_factor):
+    return np.abs(c.astype(np.float64)) <= bound_factor * np.abs(w.astype(np.float64))
+
+
+def test_visibility_bound_without_division():
+    """K1's main-view visibility -lim < RN32(c / w) < lim (renderer.cu:105-107,
+    lim = RN32(1 - RN32(1e-3))) is evaluated as |c| <= LIM_MID |w| in binary64
+    for finite non-zero w, LIM_MID = 33520877 / 2^25 the midpoint below lim.
+    Checked on random operands over wide exponent ranges (subnormal w
+    included) and on c within two ulps of the bound, where a wrong bound or an
+    inexact product shows (using lim itself as the bound fails here)."""
+    lim = np.float32(1.0) - np.float32(1e-3)
+    lim_mid = 33520877.0 / 33554432.0
+    assert (float(lim) + float(np.nextafter(lim, np.float32(0)))) / 2 == lim_mid
+    rng = np.random.default_rng(12)
+    n = 400_000
+    w = (rng.uniform(1.0, 2.0, n) * 2.0 ** rng.integers(-140, 120, n)).astype(np.float32)
+    w *= np.where(rng.random(n) < 0.5, -1, 1).astype(np.float32)
+    w = w[(w != 0) & np.isfinite(w)]
+    up, dn = np.float32(np.inf), np.float32(-np.inf)
+    near = (lim_mid * w.astype(np.float64)).astype(np.float32)  # nearest binary32 to the bound
+    cands = {"near": near, "up1": np.nextafter(near, up), "dn1": np.nextafter(near, dn)}
+    cands["up2"] = np.nextafter(cands["up1"], up)
+    cands["dn2"] = np.nextafter(cands["dn1"], dn)
+    cands["random"] = (w.astype(np.float64) * rng.uniform(-2.0, 2.0, w.size)).astype(np.float32)
+    mixed = 0
+    with np.errstate(over="ignore", under="ignore", invalid="ignore", divide="ignore"):
+        for sign in (np.float32(1), np.float32(-1)):
+            for name, c in cands.items():
+                c = c * sign
+                ok = np.isfinite(c)
+                cc, ww = c[ok], w[ok]
+                q = cc / ww  # binary32 IEEE division
+                ref = (q > -lim) & (q < lim)
+                assert np.array_equal(ref, _visible_fast(cc, ww, lim_mid)), name
+                mixed += bool(ref.any() and not ref.all())
+                if name == "near":  # the wrong bound (lim itself) differs here
+                    assert not np.array_equal(ref, _visible_fast(cc, ww, float(lim)))
+    assert mixed >= 4  # the near-bound sets hold both outcomes
