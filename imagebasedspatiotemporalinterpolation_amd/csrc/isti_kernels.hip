@@ -100,7 +100,7 @@ __device__ __forceinline__ bool project_pixel(const J_t &J, const M_t &vp_cam, c
     ty = y;
     if (!init_linear) {
         const float4 cs_cam = project(vp_cam, ws_scaled);
-        J.cam_ff[p] = make_float4(cs_cam.x - cs_from.x, cs_cam.y - cs_from.y, cs_cam.z - cs_from.z, vis);
+        J.cam_ff[p] = make_float4(cs_cam.x - cs_from.x, cs_cam.y - cs_from.y, z_from + (cs_cam.z - cs_from.z), vis);
         const float depth_cam = z_to_depth(cs_cam.z);
         const int2 pc = cs2pixel(cs_cam.x, cs_cam.y, J.w, J.h);
         if (!in_range(pc, J.w, J.h) || depth_cam < 0.0f) return false;
@@ -108,7 +108,7 @@ __device__ __forceinline__ bool project_pixel(const J_t &J, const M_t &vp_cam, c
         tx = (uint32_t)pc.x;
         ty = (uint32_t)pc.y;
     } else {
-        J.cam_ff[p] = make_float4(fx * wt, fy * wt, fz * wt, vis);
+        J.cam_ff[p] = make_float4(fx * wt, fy * wt, z_from + fz * wt, vis);
     }
     key = encode_key(depth, x, y);
     return true;
@@ -217,8 +217,9 @@ __global__ __launch_bounds__(256) void features_kernel(const FeatureArgs a) {
         f[0] = co.x / 255.0f; f[1] = co.y / 255.0f; f[2] = co.z / 255.0f;
         f[3] = cq.x / 255.0f; f[4] = cq.y / 255.0f; f[5] = cq.z / 255.0f;
         f[6] = own.z[p]; f[7] = oth.z[p];
-        f[8] = mo.x * hw; f[9] = mo.y * hh; f[10] = mo.z;
-        f[11] = mq.x * hw; f[12] = mq.y * hh; f[13] = mq.z;
+        // z motion back from the moved z the records hold
+        f[8] = mo.x * hw; f[9] = mo.y * hh; f[10] = mo.z - f[6];
+        f[11] = mq.x * hw; f[12] = mq.y * hh; f[13] = mq.z - f[7];
         f[14] = d == 0 ? a.alpha : 1.0f - a.alpha;
     }
     uint16_t *o = a.x + (size_t)i * 16;

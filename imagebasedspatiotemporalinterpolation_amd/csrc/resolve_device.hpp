@@ -506,7 +506,7 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
             si[c][d] = SplatInfo{0.0f, qx, qy};
             if (ran[c][d]) {
                 const float4 m = at(v.cam_ff, v.w, qx, qy);
-                si[c][d].depth = z_to_depth(at(v.z, v.w, qx, qy) + m.z);
+                si[c][d].depth = z_to_depth(m.z);  // m.z = z + flow z (the splat's sum)
                 err[c][d] = reproj_error(make_uint2(qx, qy), v, cpcs[c], m);
                 vis[c][d] = m.w > 0.0f;
             }
@@ -546,7 +546,7 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
                 const float4 m = at(v.cam_ff, v.w, qx, qy);
                 si[0][d].x = qx;
                 si[0][d].y = qy;
-                si[0][d].depth = z_to_depth(at(v.z, v.w, qx, qy) + m.z);
+                si[0][d].depth = z_to_depth(m.z);
                 err[0][d] = reproj_error(make_uint2(qx, qy), v, cpcs[0], m);
                 vis[0][d] = m.w > 0.0f;
             }

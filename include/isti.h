@@ -132,9 +132,12 @@ typedef struct isti_renderer isti_renderer;
 
 /* Device scratch of one (slot, view) after the last render, for inspection:
  * the u64 splat buffer and the two flow planes, all device pointers, rows
- * packed: cam_ff (source -> target camera) holds 4 floats per pixel (x, y, z,
- * visible-in-main flag), full_ff (source -> other keyframe) 3 floats (x, y,
- * z; the reference's fourth component repeats the flag and is never read).
+ * packed: cam_ff (source -> target camera) holds 4 floats per pixel: flow x,
+ * y, the moved NDC z = RN32(z + flow z) (the reference keeps the flow z and
+ * adds the keyframe z where it reads it, renderer.cu:338-341; the sum is
+ * stored instead, so a resolve needs no z fetch) and the visible-in-main flag;
+ * full_ff (source -> other keyframe) 3 floats (x, y, z; the reference's
+ * fourth component repeats the flag and is never read).
  * Reference: Renderer members, include/renderer.h:64-92. */
 typedef struct isti_scratch_view {
     uint32_t width;

@@ -34,6 +34,9 @@ def _check(O, cfg, fwd, bwd, b, mode=0, max_iter=-1, no_shadows=False, alpha=Non
             assert F.flow_err_px(g["cam_ff"], s["cam_ff"]) <= 1e-3, f"cam_ff {key}"
             assert F.flow_err_px(g["full_ff"], s["full_ff"]) <= 1e-3, f"full_ff {key}"
             np.testing.assert_array_equal(g["cam_ff"][..., 3], s["cam_ff"][..., 3])
+            # the device record holds the moved z, RN32(z + flow z), where the oracle keeps the flow z
+            z = (fwd if key[0] == 0 else bwd)[key[1]]["z"]
+            np.testing.assert_array_equal(g["cam_ff"][..., 2], z + s["cam_ff"][..., 2], err_msg=f"cam_ff z {key}")
     F.assert_rgb_parity(got, ref)
     return got, ref
 
