@@ -140,6 +140,9 @@ def stereo_bench(n_frames: int, seed: int) -> dict:
     stream = torch.cuda.current_stream()
 
     def frame(i):
+        if i % len(infos) == 0:  # a new keyframe pair: its flows are computed once more
+            for r, fd, _ in eyes:
+                r.set_frame(fd)
         batch.render([(eyes[e][0], infos[i % len(infos)][e], eyes[e][2]) for e in range(2)], stream=stream)
 
     for i in range(10):
@@ -197,19 +200,24 @@ def learned_bench(n_frames: int, seed: int, iters: int) -> dict:
     stream = torch.cuda.current_stream()
 
     def timed(fn):
-        for i in range(10):
+        def step(i):
+            if i % len(infos) == 0:  # a new keyframe pair: its flows are computed once more
+                r.set_frame(fd)
             fn(infos[i % len(infos)])
+
+        for i in range(10):
+            step(i)
         torch.cuda.synchronize()
         gc.disable()  # no collector pauses between the enqueued frames
         t0 = time.perf_counter()
         for i in range(n_frames):
-            fn(infos[i % len(infos)])
+            step(i)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n_frames)]
         for i in range(n_frames):  # latency pass
             ev[i][0].record(stream)
-            fn(infos[i % len(infos)])
+            step(i)
             ev[i][1].record(stream)
         torch.cuda.synchronize()
         gc.enable()
@@ -297,7 +305,21 @@ def main():
         if st != 0:
             N.check(st, h)
 
+    # A step is one keyframe pair: its keyframes are set at the start of the
+    # step, so every step computes the pair's flow toward the other keyframe
+    # once (the renderer keeps it for the pair's later frames).
+    pair_descs = [(0, 0, fd.fwd.desc(0)), (1, 0, fd.bwd.desc(0))]
+    for i, a in enumerate(fd.aux_cam_images):
+        pair_descs += [(0, i + 1, a.curr.desc(a.priority)), (1, i + 1, a.next.desc(a.priority))]
+
+    def new_pair():
+        for slot, view, d in pair_descs:
+            st = L.isti_keyframe_set(h, slot, view, ctypes.byref(d))
+            if st != 0:
+                N.check(st, h)
+
     for _ in range(args.warmup):
+        new_pair()
         for ci in infos:
             frame(ci)
     torch.cuda.synchronize()
@@ -320,6 +342,7 @@ def main():
     gc.disable()
     t0 = time.perf_counter()
     for _ in range(args.steps):
+        new_pair()
         for ci in infos:
             frame(ci)
     torch.cuda.synchronize()
@@ -330,6 +353,7 @@ def main():
     gc.disable()
     k = 0
     for _ in range(args.steps):
+        new_pair()
         for ci in infos:
             ev[k][0].record(stream)
             frame(ci)
@@ -341,6 +365,7 @@ def main():
     # 3. per-stage times from the renderer's own events on its stream
     rend.set_stage_timing(True)
     for _ in range(max(1, min(args.steps, 50))):
+        new_pair()
         for ci in infos:
             frame(ci)
     torch.cuda.synchronize()

@@ -45,6 +45,10 @@ struct Scratch {
     unsigned long long *splat = nullptr;
     float4 *cam_ff = nullptr;
     Flow3 *full_ff = nullptr;
+    // full_ff holds the flow of the keyframe pair currently set (it depends
+    // on the pair alone); cleared when either view of the pair is set, on a
+    // swap and when the buffers are reallocated
+    bool full_ok = false;
 };
 
 // Ring of per-render event sets for stage timing. A slot is harvested (its
@@ -322,6 +326,7 @@ isti_status plan_frame(isti_renderer *r, const isti_render_info *info, uint8_t *
             J.main_slot = sl;
             J.inv_w = 1.0 / (double)d.width;
             J.rcp_w = (float)(1.0 / (double)d.width);
+            J.write_full = sc.full_ok ? 0 : 1;
             J.inv_h = 1.0 / (double)d.height;
             J.inv_vp_from = to_mat(d.inv_viewproj);
             J.vp_to = to_mat(other.viewproj);
@@ -386,6 +391,12 @@ bool splat_binned_for(const isti_renderer *r, const FramePlan &P) {
 
 // M1 clear and K1 reprojection + splat of every view of a planned frame
 // (src/renderer.cu:631-649), with the stage-timing marks 0, 1 and 2.
+// After a splat launch: full_ff of every view holds the current pair's flow.
+void mark_full_written(isti_renderer *r) {
+    for (int sl = 0; sl < 2; sl++)
+        for (int v = 0; v <= r->n_aux; v++) r->scratch[sl][v].full_ok = true;
+}
+
 isti_status splat_stage(isti_renderer *r, const FramePlan &P, hipStream_t s, bool binned) {
     ClearArgs ca;
     std::memset(&ca, 0, sizeof(ca));
@@ -415,6 +426,7 @@ isti_status splat_stage(isti_renderer *r, const FramePlan &P, hipStream_t s, boo
     if ((e = launch_clear(ca, s)) != hipSuccess) return dev_fail(r, e, "clear launch");
     if ((e = mark(r, 1, s)) != hipSuccess) return dev_fail(r, e, "event");
     if ((e = launch_splat(sa, binned, s)) != hipSuccess) return dev_fail(r, e, "splat launch");
+    mark_full_written(r);
     if ((e = mark(r, 2, s)) != hipSuccess) return dev_fail(r, e, "event");
     return ISTI_OK;
 }
@@ -659,6 +671,7 @@ isti_status isti_render_batch(isti_batch *b, isti_renderer *const *rs, const ist
         e = launch_batch_stages(b, td, mask, s);
     }
     if (e != hipSuccess) return bfail(b, ISTI_ERR_DEVICE, std::string("batch launch: ") + hipGetErrorString(e));
+    for (int i = 0; i < n; i++) mark_full_written(rs[i]);
     e = hipEventRecord(b->done[sl], s);
     if (e != hipSuccess) return bfail(b, ISTI_ERR_DEVICE, hipGetErrorString(e));
     b->in_use[sl] = true;
@@ -825,6 +838,10 @@ isti_status isti_keyframe_set(isti_renderer *r, int32_t slot, int32_t view, cons
     if (nm != 0 && nm != 3) return fail(r, ISTI_ERR_INVALID_ARG, "motion planes: all three or none");
     r->views[slot][view] = *d;
     r->view_set[slot][view] = true;
+    // the flows of this view and of its partner in the other slot (whose
+    // target this view is) must be recomputed
+    r->scratch[0][view].full_ok = false;
+    r->scratch[1][view].full_ok = false;
     return ISTI_OK;
 }
 
@@ -834,6 +851,7 @@ isti_status isti_keyframe_swap(isti_renderer *r) {
     for (int v = 0; v < MAX_VIEWS; v++) {
         std::swap(r->views[0][v], r->views[1][v]);
         std::swap(r->view_set[0][v], r->view_set[1][v]);
+        r->scratch[0][v].full_ok = r->scratch[1][v].full_ok = false;
     }
     return ISTI_OK;
 }

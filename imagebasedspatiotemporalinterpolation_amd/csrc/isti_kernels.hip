@@ -91,9 +91,18 @@ __device__ __forceinline__ bool project_pixel(const J_t &J, const M_t &vp_cam, c
     const float4 ws_scaled = make_float4(__builtin_isfinite(wt) ? sx : __builtin_nanf(""),
                                          ws_from.y + wt * (ws_to.y - ws_from.y),
                                          ws_from.z + wt * (ws_to.z - ws_from.z), 1.0f);
-    const float4 cs_to = project(J.vp_to, ws_to);
-    const float fx = cs_to.x - cs_from.x, fy = cs_to.y - cs_from.y, fz = cs_to.z - cs_from.z;
-    J.full_ff[p] = Flow3{fx, fy, fz};
+    // The flow toward the other keyframe depends on the keyframe pair alone
+    // (not on the render camera or alpha): the first render of a pair writes
+    // it and later renders keep it (write_full is uniform per job). The
+    // linear initialisation scales it into cam_ff, so it computes it anyway.
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    if (J.write_full || init_linear) {
+        const float4 cs_to = project(J.vp_to, ws_to);
+        fx = cs_to.x - cs_from.x;
+        fy = cs_to.y - cs_from.y;
+        fz = cs_to.z - cs_from.z;
+        if (J.write_full) J.full_ff[p] = Flow3{fx, fy, fz};
+    }
 
     float depth = z_to_depth(cs_from.z);
     tx = x;

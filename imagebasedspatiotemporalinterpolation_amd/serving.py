@@ -246,6 +246,9 @@ class Server:
             return
         for r in chosen:
             s = self.sessions[r.sid]
+            if r.b == 1:  # the first frame of a keyframe pair: the pair's flows are computed once more
+                for rend, fd in zip(s.renderers, s.frames):
+                    rend.set_frame(fd)
             outs = r.outs if r.outs is not None else s.outs
             for e, rend in enumerate(s.renderers):
                 work.append((rend, s.info(r.b, e, self.max_iter), outs[e]))

@@ -156,7 +156,14 @@ isti_status isti_destroy(isti_renderer *r);
 
 /* Sets one keyframe view of a slot (the reference loads FrameData through
  * FrameFileReader, src/file_reader.cu:138-217; here the caller hands over
- * device planes). view: 0 = main, 1..n_aux = aux camera view-1. */
+ * device planes). view: 0 = main, 1..n_aux = aux camera view-1.
+ * Every render reads the planes afresh, except the flow of a view toward its
+ * partner keyframe (the other slot's same view), which depends on the pair
+ * alone: the first render after this view or its partner was set (or after a
+ * swap) computes it, and later renders reuse it. A caller that rewrites a
+ * keyframe's planes in place sets the view again before the next render (the
+ * reference's render takes its FrameData on every call; setting every view
+ * before every render reproduces that exactly). */
 isti_status isti_keyframe_set(isti_renderer *r, int32_t slot, int32_t view,
                               const isti_view_desc *desc);
 

@@ -91,9 +91,18 @@ public:
     Renderer &operator=(const Renderer &) = delete;
 
     // Renderer::render, include/renderer.h:56 / src/renderer.cu:607-694.
-    // Stream-ordered; returns without waiting for the GPU.
+    // Stream-ordered; returns without waiting for the GPU. Sets the keyframes
+    // on every call, as the reference takes its FrameData on every call.
     void render(RenderBuffer &render_buffer, const RenderInfo &info, const FrameData &frame_data,
                 void *stream = nullptr) {
+        set_frame(frame_data);
+        render(render_buffer, info, stream);
+    }
+
+    // Sets the keyframe pair once for the renders that follow; those reuse
+    // the pair's flow toward the other keyframe (computed by the first of
+    // them). Call again when the pair, or a plane's contents, change.
+    void set_frame(const FrameData &frame_data) {
         set(ISTI_SLOT_FWD, 0, frame_data.fwd, 0);
         set(ISTI_SLOT_BWD, 0, frame_data.bwd, 0);
         for (size_t i = 0; i < frame_data.aux_cam_images.size(); i++) {
@@ -101,6 +110,10 @@ public:
             set(ISTI_SLOT_FWD, (int)i + 1, a.curr, a.priority);
             set(ISTI_SLOT_BWD, (int)i + 1, a.next, a.priority);
         }
+    }
+
+    // A render of the pair last passed to set_frame.
+    void render(RenderBuffer &render_buffer, const RenderInfo &info, void *stream = nullptr) {
         isti_render_info ri;
         std::memset(&ri, 0, sizeof(ri));
         ri.width = info.resolution[0];

@@ -1,12 +1,17 @@
 """Successive frames on one renderer: the scratch (splat buffers, flows)
 carried from frame to frame must leave every frame with the oracle's splat
-buffers and pixels, also when the path changes between frames (plain
-render, autotune, batch, learned-start resolve, window-3 mode) and when the
-keyframe size changes (the scratch is reallocated)."""
+buffers, flows toward the other keyframe and pixels, also when the path
+changes between frames (plain render, autotune, batch, learned-start
+resolve, window-3 mode), when the keyframe size changes (the scratch is
+reallocated), and when keyframes change at the same size (the flow toward
+the other keyframe is kept per pair and must be recomputed)."""
+import ctypes
+
 import numpy as np
 import pytest
 import torch
 
+from imagebasedspatiotemporalinterpolation_amd import _native as N
 from imagebasedspatiotemporalinterpolation_amd import learned as Ln
 from imagebasedspatiotemporalinterpolation_amd import renderer as R
 from tests import _frames as F
@@ -28,9 +33,17 @@ class _Seq:
         r.set_frame(self.fd)
         self.fpa = cfg.keyframe_spacing
         self.sfpa = cfg.shadow_frames if cfg.shadows else 0
+        self.k = 0
+
+    def use_pair(self, k):
+        """Keyframe pair k, set on the renderer (all views)."""
+        self.fwd, self.bwd = F.pair(self.cfg, k)
+        self.fd = _frame_data(self.fwd, self.bwd)
+        self.r.set_frame(self.fd)
+        self.k = k
 
     def info(self, b, mode=0, iters=8):
-        vp, ivp, w, h = F.target(self.cfg, 0, b)
+        vp, ivp, w, h = F.target(self.cfg, self.k, b)
         ri = R.RenderInfo((w, h), b / self.fpa, vp, ivp,
                           R.AdditionalRenderInfo(self.fpa, self.sfpa, False, R.RenderMode(mode)), iters)
         return ri, vp, ivp, w, h
@@ -49,8 +62,10 @@ class _Seq:
         ref, sref = F.oracle_frame(self.O, self.fwd, self.bwd, vp, ivp, b / self.fpa, self.fpa, self.sfpa, mode,
                                    iters, False, prev)
         for key, sc in sref.items():
-            got = self.r.scratch(*key)["splat"]
+            g = self.r.scratch(*key)
+            got = g["splat"]
             assert np.array_equal(got, sc["splat"]), (what, key, int((got != sc["splat"]).sum()))
+            assert F.flow_err_px(g["full_ff"], sc["full_ff"]) <= 1e-3, (what, key, "full_ff")
         F.assert_rgb_parity(out.cpu().numpy(), ref)
 
 
@@ -117,3 +132,68 @@ def test_sequence_across_paths_and_sizes(oracle_lib):
     batch.close()
     r.close()
 
+
+
+def _set_view(r, slot, view, kv, priority):
+    d = kv.desc(priority)
+    N.check(N.lib().isti_keyframe_set(r._h, slot, view, ctypes.byref(d)), r._h)
+
+
+def test_pair_flow_reused_until_a_keyframe_changes(oracle_lib):
+    """The flow toward the other keyframe depends on the keyframe pair alone:
+    the first render after a keyframe is set computes it, later renders of the
+    pair reuse it. Every frame is checked against the oracle (splat buffers,
+    that flow, pixels) across frames of one pair, a new pair of the same size,
+    one view's partner set alone, a plane rewritten in place and set again, a
+    swap, the batch path and the linear-initialisation mode."""
+    cfg = F.scaled(2, 7, 4)
+    r = R.Renderer(n_aux=2)
+    S = _Seq(oracle_lib, cfg, r)
+    batch = R.Batch(max_requests=2)
+    for b in (1, 2, 3):
+        print(f"case: pair 0 frame {b}", flush=True)
+        S.check(S.render(b), b, what=f"pair 0 frame {b}")
+
+    S.use_pair(1)
+    for b in (1, 2):
+        print(f"case: pair 1 frame {b}", flush=True)
+        S.check(S.render(b), b, what=f"pair 1 frame {b}")
+
+    # the bwd main view alone (the partner of the fwd main view's flow)
+    print("case: partner set alone", flush=True)
+    other = F.pair(cfg, 2)[1][0]
+    kv = R.KeyframeView.from_numpy(other)
+    _set_view(r, 1, 0, kv, 0)
+    S.bwd = [other] + S.bwd[1:]
+    S.check(S.render(3), 3, what="partner set alone")
+    S.check(S.render(1), 1, what="partner set alone, next frame")
+
+    # a plane rewritten in place, then the view set again
+    print("case: plane rewritten in place", flush=True)
+    z2 = F.pair(cfg, 3)[1][0]["z"]
+    kv.z.copy_(torch.from_numpy(z2).cuda())
+    _set_view(r, 1, 0, kv, 0)
+    S.bwd = [dict(other, z=z2)] + S.bwd[1:]
+    S.check(S.render(2), 2, what="rewritten in place")
+
+    print("case: swap", flush=True)
+    r.swap()
+    S.fwd, S.bwd = S.bwd, S.fwd
+    S.check(S.render(2), 2, what="after swap")
+    S.check(S.render(4), 4, what="after swap, next frame")
+
+    print("case: batch", flush=True)
+    S.use_pair(0)
+    for b in (3, 4):
+        ri, _, _, w, h = S.info(b)
+        out = torch.zeros((h, w, 4), dtype=torch.uint8, device="cuda")
+        batch.render([(r, ri, out)])
+        S.check(out, b, what=f"batch frame {b}")
+
+    print("case: linear initialisation after reuse", flush=True)
+    mode = int(R.RenderMode.BACKWARD_BIDIRECTIONAL)
+    S.check(S.render(5, mode), 5, mode, what="linear init")
+    S.check(S.render(2), 2, what="plain after linear init")
+    del kv
+    batch.close()
+    r.close()
