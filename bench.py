@@ -404,6 +404,8 @@ def main():
             "workload": "1920x1080 main + 2 aux 960x540 (+-0.3 m), soft shadows (8 sub-frames), "
                         "7 interpolated frames per keyframe pair (fpa 8), mode 0, plain search <= 32 iterations",
             "frames_per_step": len(infos),
+            "step": "one keyframe pair: its keyframes are set, then its 7 frames rendered "
+                    "(the pair's flow toward the other keyframe is computed by the first)",
             "search": "plain",
             "max_search_iterations": args.iters,
             "mode": args.mode,
