@@ -1,0 +1,42 @@
+"""bench.py's contract pieces that need no GPU: it refuses more than one GPU
+with a message, the algorithmic bytes behind roofline.achieved match the
+per-unit figures (clear 8 B per splat cell, splat 24 B per source pixel,
+resolve 12 B per source pixel, +4 with shadows, and 4 B per output pixel)
+for the 1080p + 2 aux workload, and the traffic file it reads names every
+stage."""
+import json
+import subprocess
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def test_more_than_one_gpu_is_refused():
+    p = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120, cwd=ROOT)
+    assert p.returncode != 0
+    assert "one GPU" in p.stdout + p.stderr
+    assert "{" not in p.stdout  # no JSON line
+
+
+def test_algorithmic_bytes_of_the_1080p_workload():
+    sys.path.insert(0, str(ROOT))
+    import bench
+
+    main, aux = 1920 * 1080, 960 * 540
+    src = 2 * (main + 2 * aux)  # fwd and bwd keyframe of every view
+    assert src == 6_220_800
+    b = bench.algorithmic_bytes(src, main, True)
+    assert b["clear"] == 49_766_400
+    assert b["splat"] == 149_299_200
+    assert b["resolve"] == 16 * src + 4 * main == 107_827_200
+    assert bench.algorithmic_bytes(src, main, False)["resolve"] == 12 * src + 4 * main
+
+
+def test_traffic_file_names_every_stage():
+    t = json.loads((ROOT / "profiles" / "traffic_1080p.json").read_text())
+    for k in ("clear", "splat", "resolve"):
+        assert t[k] > 0
+    # measured bytes are at least the compulsory ones
+    assert t["clear"] >= 49_766_400 * 0.99 and t["resolve"] >= 107_827_200
