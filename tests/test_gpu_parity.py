@@ -129,6 +129,27 @@ def test_c3_small_stereo_eye(oracle_lib):
         F.assert_rgb_parity(got, ref)
 
 
+def test_c3_full_stereo_eyes(oracle_lib):
+    """Config C3 at its real shape: both 2064x2208 eyes of a fast-camera frame
+    (0.15 m + 2 degrees per frame, 64 moving spheres), splat buffers, flows
+    and pixels."""
+    cfg = F.scaled(3, 4)
+    fpa = cfg.keyframe_spacing
+    for eye in (0, 1):
+        fwd, bwd = F.pair(cfg, 0, eye)
+        vp, ivp, w, h = F.target(cfg, 0, 3, eye)
+        prev = _prev(h, w)
+        print(f"case: C3 eye {eye} {w}x{h}", flush=True)
+        ref, sref = F.oracle_frame(oracle_lib, fwd, bwd, vp, ivp, 3 / fpa, fpa, 0, prev=prev)
+        got, sgot = F.gpu_frame(fwd, bwd, vp, ivp, 3 / fpa, fpa, 0, prev=prev)
+        for key, s in sref.items():
+            g = sgot[key]
+            np.testing.assert_array_equal(g["splat"], s["splat"], err_msg=f"splat {key}")
+            assert F.flow_err_px(g["cam_ff"], s["cam_ff"]) <= 1e-3, f"cam_ff {key}"
+            assert F.flow_err_px(g["full_ff"], s["full_ff"]) <= 1e-3, f"full_ff {key}"
+        F.assert_rgb_parity(got, ref)
+
+
 def test_c2_full_1080p(oracle_lib):
     """Full-size workload: 1920x1080, 2 aux 960x540, shadows, search cap 32."""
     cfg = F.scaled(2, 3)
