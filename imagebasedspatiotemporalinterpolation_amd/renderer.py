@@ -164,6 +164,10 @@ class Renderer:
             pass
 
     def set_frame(self, frame_data: FrameData) -> None:
+        """Hands the keyframe pair to the renderer for the renders that follow.
+        They read the planes afresh, except the pair's flow toward the other
+        keyframe, which the first of them computes and the others reuse; call
+        again when the pair or a plane's contents change."""
         L = N.lib()
         if len(frame_data.aux_cam_images) != self.n_aux:
             raise ValueError(f"renderer expects {self.n_aux} aux cameras")
@@ -176,7 +180,9 @@ class Renderer:
 
     def render(self, render_buffer, render_info: RenderInfo, frame_data: Optional[FrameData] = None,
                stream=None) -> None:
-        """Renderer::render (src/renderer.cu:607-694); stream-ordered, no host sync."""
+        """Renderer::render (src/renderer.cu:607-694); stream-ordered, no host sync.
+        With frame_data the keyframes are set first (everything recomputed, as
+        the reference); without it the pair last set is rendered."""
         import torch
 
         if frame_data is not None:
