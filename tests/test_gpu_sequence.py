@@ -190,6 +190,19 @@ def test_pair_flow_reused_until_a_keyframe_changes(oracle_lib):
         batch.render([(r, ri, out)])
         S.check(out, b, what=f"batch frame {b}")
 
+    # replayed stage graphs across a pair change: the per-job flow flag
+    # travels in the uploaded work table, not in the graph
+    print("case: batch with stage graphs", flush=True)
+    gbatch = R.Batch(max_requests=2, graph=True)
+    for k, b in ((0, 5), (0, 6), (1, 1), (1, 2)):
+        if k != S.k:
+            S.use_pair(k)
+        ri, _, _, w, h = S.info(b)
+        out = torch.zeros((h, w, 4), dtype=torch.uint8, device="cuda")
+        gbatch.render([(r, ri, out)])
+        S.check(out, b, what=f"graph batch pair {k} frame {b}")
+    gbatch.close()
+
     print("case: linear initialisation after reuse", flush=True)
     mode = int(R.RenderMode.BACKWARD_BIDIRECTIONAL)
     S.check(S.render(5, mode), 5, mode, what="linear init")
