@@ -8,6 +8,7 @@ generator emits (see imagebasedspatiotemporalinterpolation_amd/scene.py).
 from __future__ import annotations
 
 import ctypes
+import os
 from ctypes import POINTER, c_float, c_int, c_int32, c_uint32, c_uint64, c_void_p
 from pathlib import Path
 
@@ -15,7 +16,9 @@ import numpy as np
 
 from imagebasedspatiotemporalinterpolation_amd._native import RenderInfoC, ViewDesc, mat16
 
-LIB_PATH = Path(__file__).resolve().parent / "liboracle.so"
+# ISTI_ORACLE_LIB points the binding at another build of oracle.cpp (the
+# mutation tests load deliberately broken copies from a temporary directory).
+LIB_PATH = Path(os.environ.get("ISTI_ORACLE_LIB") or Path(__file__).resolve().parent / "liboracle.so")
 
 
 class OracleScratch(ctypes.Structure):
