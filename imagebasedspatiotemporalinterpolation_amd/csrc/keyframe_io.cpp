@@ -93,7 +93,7 @@ isti_status decode_png(const std::string &path, uint32_t &w, uint32_t &h, std::v
         p += 12 + (size_t)len;
     }
     const int nch = ctype == 0 ? 1 : ctype == 4 ? 2 : ctype == 2 ? 3 : ctype == 6 ? 4 : 0;
-    if (w == 0 || h == 0 || w > ISTI_MAX_DIM || h > ISTI_MAX_DIM)
+    if (w == 0 || h == 0 || w > ISTI_MAX_DIM || h > ISTI_MAX_DIM || (uint64_t)w * h >= ISTI_MAX_VIEW_PIXELS)
         return fail(ISTI_ERR_UNSUPPORTED_SHAPE, path + ": bad size");
     if (depth != 8 || nch == 0 || interlace != 0)
         return fail(ISTI_ERR_INVALID_ARG, path + ": only 8-bit grey/grey-alpha/RGB/RGBA non-interlaced PNG");
@@ -204,7 +204,7 @@ isti_status decode_exr(const std::string &path, const std::vector<std::string> &
         if (!cstr(type) || p + 4 > f.size()) return fail(ISTI_ERR_INVALID_ARG, path + ": truncated header");
         const int32_t size = rd32(p);
         p += 4;
-        if (size < 0 || p + (size_t)size > f.size()) return fail(ISTI_ERR_INVALID_ARG, path + ": bad attribute");
+        if (size < 0 || (size_t)size > f.size() - p) return fail(ISTI_ERR_INVALID_ARG, path + ": bad attribute");
         const size_t end = p + (size_t)size;
         if (name == "channels" && type == "chlist") {
             size_t q = p;
@@ -220,6 +220,7 @@ isti_status decode_exr(const std::string &path, const std::vector<std::string> &
                 q = e + 17;
             }
         } else if (name == "compression") {
+            if (size < 1) return fail(ISTI_ERR_INVALID_ARG, path + ": bad compression attribute");
             compression = f[p];
         } else if (name == "dataWindow" && size == 16) {
             xmin = rd32(p);
@@ -234,9 +235,11 @@ isti_status decode_exr(const std::string &path, const std::vector<std::string> &
         names->clear();
         for (const ExrChannel &c : chans) names->push_back(c.name);
     }
-    w = (uint32_t)(xmax - xmin + 1);
-    h = (uint32_t)(ymax - ymin + 1);
-    if (w > ISTI_MAX_DIM || h > ISTI_MAX_DIM) return fail(ISTI_ERR_UNSUPPORTED_SHAPE, path + ": too large");
+    const int64_t w64 = (int64_t)xmax - xmin + 1, h64 = (int64_t)ymax - ymin + 1;
+    if (w64 > ISTI_MAX_DIM || h64 > ISTI_MAX_DIM || w64 * h64 >= (int64_t)ISTI_MAX_VIEW_PIXELS)
+        return fail(ISTI_ERR_UNSUPPORTED_SHAPE, path + ": too large");
+    w = (uint32_t)w64;
+    h = (uint32_t)h64;
     int lines;
     switch (compression) {
         case 0:
@@ -260,16 +263,18 @@ isti_status decode_exr(const std::string &path, const std::vector<std::string> &
     }
     out->assign((size_t)w * h * nw, 0.0f);
     const size_t n_chunks = (h + (uint32_t)lines - 1) / (uint32_t)lines;
-    if (p + 8 * n_chunks > f.size()) return fail(ISTI_ERR_INVALID_ARG, path + ": truncated offsets");
+    if (8 * n_chunks > f.size() - p) return fail(ISTI_ERR_INVALID_ARG, path + ": truncated offsets");
     std::vector<uint8_t> raw, tmp;
     for (size_t k = 0; k < n_chunks; k++) {
         uint64_t off;
         std::memcpy(&off, &f[p + 8 * k], 8);
-        if (off + 8 > f.size()) return fail(ISTI_ERR_INVALID_ARG, path + ": bad chunk offset");
-        const int32_t y0 = rd32((size_t)off) - ymin;
+        // offsets come from the file: compare without forming off + 8 (it may wrap)
+        if (f.size() < 8 || off > f.size() - 8) return fail(ISTI_ERR_INVALID_ARG, path + ": bad chunk offset");
+        const int64_t y0_64 = (int64_t)rd32((size_t)off) - ymin;
         const int32_t n = rd32((size_t)off + 4);
-        if (y0 < 0 || (uint32_t)y0 >= h || n < 0 || off + 8 + (uint64_t)n > f.size())
+        if (y0_64 < 0 || y0_64 >= (int64_t)h || n < 0 || (uint64_t)n > f.size() - 8 - off)
             return fail(ISTI_ERR_INVALID_ARG, path + ": bad chunk");
+        const int32_t y0 = (int32_t)y0_64;
         const uint32_t nl = std::min<uint32_t>((uint32_t)lines, h - (uint32_t)y0);
         const size_t size = line_bytes * nl;
         const uint8_t *data = &f[(size_t)off + 8];
@@ -634,20 +639,39 @@ isti_status isti_io_camera_at(const float view_fwd[16], const float view_bwd[16]
     return ISTI_OK;
 }
 
-isti_status isti_io_decode_png(const char *path, uint32_t *w, uint32_t *h, uint8_t *out) {
+isti_status isti_io_decode_png(const char *path, uint32_t *w, uint32_t *h, uint8_t *out, size_t out_bytes) {
     if (!path || !w || !h) return fail(ISTI_ERR_INVALID_ARG, "path, w and h must be non-null");
     std::vector<uint8_t> img;
     const isti_status st = decode_png(path, *w, *h, out ? &img : nullptr);
-    if (st == ISTI_OK && out) std::memcpy(out, img.data(), img.size());
-    return st;
+    if (st != ISTI_OK || !out) return st;
+    if (img.size() > out_bytes)  // the file may have changed since the size query
+        return fail(ISTI_ERR_INVALID_ARG, std::string(path) + ": image does not fit the output buffer");
+    std::memcpy(out, img.data(), img.size());
+    return ISTI_OK;
 }
 
-isti_status isti_io_decode_exr(const char *path, const char *channels, uint32_t *w, uint32_t *h, float *out) {
+isti_status isti_io_decode_exr(const char *path, const char *channels, uint32_t *w, uint32_t *h, float *out,
+                               size_t out_floats) {
     if (!path || !channels || !w || !h) return fail(ISTI_ERR_INVALID_ARG, "path, channels, w and h must be non-null");
     std::vector<float> img;
     const isti_status st = decode_exr(path, split_names(channels), *w, *h, out ? &img : nullptr);
-    if (st == ISTI_OK && out) std::memcpy(out, img.data(), img.size() * sizeof(float));
-    return st;
+    if (st != ISTI_OK || !out) return st;
+    if (img.size() > out_floats)
+        return fail(ISTI_ERR_INVALID_ARG, std::string(path) + ": image does not fit the output buffer");
+    std::memcpy(out, img.data(), img.size() * sizeof(float));
+    return ISTI_OK;
+}
+
+isti_status isti_io_client_camera(const float w2c[16], const float proj[16], float viewproj[16],
+                                  float inv_viewproj[16]) {
+    if (!w2c || !proj) return fail(ISTI_ERR_INVALID_ARG, "matrices must be non-null");
+    float p[16], vp[16], ivp[16];
+    normalize_proj(proj, p);
+    mat_mul(p, w2c, vp);
+    if (!mat_inverse(vp, ivp)) return fail(ISTI_ERR_INVALID_ARG, "singular view-projection");
+    if (viewproj) std::memcpy(viewproj, vp, sizeof(vp));
+    if (inv_viewproj) std::memcpy(inv_viewproj, ivp, sizeof(ivp));
+    return ISTI_OK;
 }
 
 isti_status isti_io_read_camera(const char *cam_dir, isti_io_camera *cam, isti_io_frame *frames,

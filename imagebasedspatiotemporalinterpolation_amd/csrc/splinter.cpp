@@ -1,8 +1,8 @@
 // splinter.cpp - host driver, the counterpart of the reference's
 // src/main.cu (CLI :66-80, frame loop :139-191).
 //
-//   splinter <input> <output-dir> [-w] [-r W H] [-a NAME...] [-l N] [-m 0..4]
-//            [-lh] [-ns] [-hb] [--frames N] [--iters N] [--seed S]
+//   splinter <input> <output-dir> [-c CAMDIR] [-w] [-r W H] [-a NAME...] [-l N]
+//            [-m 0..4] [-lh] [-ns] [-hb] [--frames N] [--iters N] [--seed S]
 //
 // <input> is either a keyframe recording in the reference's on-disk format
 // (a directory with main/ and one directory per -a NAME; read through
@@ -12,8 +12,11 @@
 // at 120 Hz, frames_per_anchor = 120 / serverFps and the target camera is the
 // interpolation of the keyframe views (main.cu:139-161); with a synthetic
 // scene the target camera follows the scene's analytic camera path at
-// client-frame times (the reference's -c camera path). -w writes
-// <output-dir>/NNNN.png per frame. Per-frame GPU times (hipEvents) are
+// client-frame times. -c CAMDIR reads a client camera path (CAMDIR/cam.json,
+// file_reader.cu:78-107): client frame n renders with frames[n].w2c and the
+// normalised frames[n].proj, the file's frame count is the run length and, for
+// a recording, its clientFps sets frames_per_anchor (main.cu:128-161). -w
+// writes <output-dir>/NNNN.png per frame. Per-frame GPU times (hipEvents) are
 // summarised at the end as Mpix/s and p50/p99 ms.
 
 #include <hip/hip_runtime.h>
@@ -45,6 +48,9 @@ namespace fs = std::filesystem;
     } while (0)
 
 namespace {
+
+// --dry-run: print the frame schedule, touch no device
+bool g_dry_run = false;
 
 // ---- minimal PNG writer (stored deflate blocks) ----
 uint32_t crc_table[256];
@@ -141,6 +147,7 @@ struct DevView {
 
 // Generates keyframe k of one view in direction dir and uploads it.
 void load_keyframe(const isti_scene_config &cfg, int k, int view, int dir, DevView &dv, bool colour_depth) {
+    if (g_dry_run) return;
     const size_t n = (size_t)dv.w * dv.h;
     std::vector<uint8_t> col(n * 4), si(n * 4), sa(n * 4), sb(n * 4);
     std::vector<float> z(n), mx(n), my(n), mz(n);
@@ -175,6 +182,14 @@ void load_recorded(const std::string &cam_dir, int pair, int which, DevView &dv)
         std::exit(2);
     }
     const size_t n = (size_t)dv.w * dv.h;
+    std::memcpy(dv.vp, v.viewproj, sizeof(dv.vp));
+    std::memcpy(dv.ivp, v.inv_viewproj, sizeof(dv.ivp));
+    std::memcpy(dv.view, v.view, sizeof(dv.view));
+    std::memcpy(dv.proj, v.proj, sizeof(dv.proj));
+    if (g_dry_run) {
+        isti_io_free_view(&v);
+        return;
+    }
     HIP_OK(hipMemcpy(dv.color, v.color, n * 4, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(dv.z, v.z, n * 4, hipMemcpyHostToDevice));
     dv.has_motion = v.motion_x != nullptr;
@@ -189,10 +204,6 @@ void load_recorded(const std::string &cam_dir, int pair, int which, DevView &dv)
         HIP_OK(hipMemcpy(dv.sh_a, v.shadow_color_a, n * 4, hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(dv.sh_b, v.shadow_color_b, n * 4, hipMemcpyHostToDevice));
     }
-    std::memcpy(dv.vp, v.viewproj, sizeof(dv.vp));
-    std::memcpy(dv.ivp, v.inv_viewproj, sizeof(dv.ivp));
-    std::memcpy(dv.view, v.view, sizeof(dv.view));
-    std::memcpy(dv.proj, v.proj, sizeof(dv.proj));
     isti_io_free_view(&v);
 }
 
@@ -200,6 +211,7 @@ void usage() {
     std::puts(
         "Splinter - Split-Rendering Frame Interpolation (MI355X)\n"
         "usage: splinter <recording-dir | scene:N[:SEED]> <output-dir> [options]\n"
+        "  -c, --camera-path DIR      client camera path (DIR/cam.json)\n"
         "  -w, --write-images         write PNG frames to output-dir\n"
         "  -r, --resolution W H       render resolution (sets the scene's main view size)\n"
         "  -a, --aux-dirs NAME...     aux camera directories (synthetic scene: up to 2 names)\n"
@@ -211,7 +223,9 @@ void usage() {
         "  --frames N                 client frames to render (default: scene 2 keyframe pairs,\n"
         "                             recording 15 s at 120 Hz or its length)\n"
         "  --iters N                  fixed-point search cap (default: the mode's own)\n"
-        "  --seed S                   scene seed");
+        "  --seed S                   scene seed\n"
+        "  --dry-run                  print the frame schedule (keyframe pair, alpha, whether the\n"
+        "                             renderer gets a new pair, target view-projection); no GPU");
 }
 
 }  // namespace
@@ -219,6 +233,7 @@ void usage() {
 int main(int argc, char **argv) {
     crc_init();
     std::vector<std::string> pos, aux_names;
+    std::string camera_path;
     bool write_images = false, hold_last = false, no_shadows = false, holes_black = false;
     int latency = 0, mode = 0, frames = -1, iters = -1, res_w = 0, res_h = 0;
     long long seed = -1;
@@ -232,6 +247,7 @@ int main(int argc, char **argv) {
         else if (a == "-lh" || a == "--latency-hold-last") hold_last = true;
         else if (a == "-ns" || a == "--no-shadows") no_shadows = true;
         else if (a == "-hb" || a == "--holes-black") holes_black = true;
+        else if (a == "--dry-run") g_dry_run = true;
         else if (a == "-l" || a == "--latency") { need(1); latency = std::atoi(argv[++i]); }
         else if (a == "-m" || a == "--mode") { need(1); mode = std::atoi(argv[++i]); }
         else if (a == "--frames") { need(1); frames = std::atoi(argv[++i]); }
@@ -241,8 +257,8 @@ int main(int argc, char **argv) {
         else if (a == "-a" || a == "--aux-dirs") {
             while (i + 1 < argc && argv[i + 1][0] != '-') aux_names.push_back(argv[++i]);
         } else if (a == "-c" || a == "--camera-path") {
-            std::fprintf(stderr, "-c: camera files are not supported; the synthetic scene's camera path is used\n");
-            if (i + 1 < argc) i++;
+            need(1);
+            camera_path = argv[++i];
         } else if (!a.empty() && a[0] == '-') { std::fprintf(stderr, "unknown option %s\n", a.c_str()); usage(); return 1; }
         else pos.push_back(a);
     }
@@ -318,6 +334,32 @@ int main(int argc, char **argv) {
         shadows = true;  // planes are used only where the recording has them
         total = frames > 0 ? frames : std::min(120 * 15, (n_keyframes - 1) * fpa);
     }
+    // main.cu:128-142: a client camera file fixes the frame count and, with a
+    // recording, the client rate
+    std::vector<isti_io_frame> client_frames;
+    if (!camera_path.empty()) {
+        isti_io_camera cc;
+        if (isti_io_read_camera(camera_path.c_str(), &cc, nullptr, 0) != ISTI_OK) {
+            std::fprintf(stderr, "%s\n", isti_io_last_error());
+            return 1;
+        }
+        client_frames.resize((size_t)cc.n_frames);
+        if (isti_io_read_camera(camera_path.c_str(), &cc, client_frames.data(), cc.n_frames) != ISTI_OK) {
+            std::fprintf(stderr, "%s\n", isti_io_last_error());
+            return 1;
+        }
+        if (!synthetic) {
+            isti_io_camera mc;
+            isti_io_read_camera(main_dir.c_str(), &mc, nullptr, 0);
+            if (cc.client_fps < mc.server_fps) {
+                std::fprintf(stderr, "camera path clientFps %d is below the recording's serverFps %d\n", cc.client_fps,
+                             mc.server_fps);
+                return 1;
+            }
+            fpa = cc.client_fps / mc.server_fps;
+        }
+        total = frames > 0 ? std::min(frames, (int)client_frames.size()) : (int)client_frames.size();
+    }
     const fs::path out_dir = pos[1];
     if (!fs::exists(out_dir) && !fs::create_directories(out_dir)) {
         std::fprintf(stderr, "Output directory could not be created\n");
@@ -336,14 +378,21 @@ int main(int argc, char **argv) {
     for (int e = 0; e < n_eyes; e++) {
         Eye &E = eyes[e];
         for (int s = 0; s < 2; s++) {
-            E.slot[s].alloc(W, H, shadows);
             E.aux[s].resize(n_aux);
+            if (g_dry_run) {  // sizes only: the loads check them
+                E.slot[s].w = W;
+                E.slot[s].h = H;
+                for (int a = 0; a < n_aux; a++) E.aux[s][a].w = AW, E.aux[s][a].h = AH;
+                continue;
+            }
+            E.slot[s].alloc(W, H, shadows);
             for (int a = 0; a < n_aux; a++) E.aux[s][a].alloc(AW, AH, shadows);
         }
+        E.fd = isti::FrameData(n_aux);
+        if (g_dry_run) continue;
         HIP_OK(hipMalloc(&E.out, (size_t)W * H * 4));
         HIP_OK(hipMemset(E.out, 0, (size_t)W * H * 4));
         E.renderer = new isti::Renderer(n_aux);
-        E.fd = isti::FrameData(n_aux);
     }
     int next_key = 0;  // keyframe index that will be read next
     int fwd_slot = 0;  // which DevView slot is currently fwd
@@ -371,9 +420,21 @@ int main(int argc, char **argv) {
         }
     }
     next_key = 1;
+    // a renderer is handed the keyframe pair once per pair (its planes are
+    // rewritten in place by read_next), so the pair's flow toward the other
+    // keyframe is computed by its first render and reused by the others;
+    // every read of a new pair clears these flags
+    std::vector<char> pair_set((size_t)n_eyes, 0);
     // file_reader.cu:167-217: swap, read the next keyframe into bwd, re-read
     // the (new) fwd keyframe's forward motion / shadow data
+    // With -l N the reference renders its first N client frames before any
+    // keyframe pair has been read, from a fwd image that was never filled
+    // (main.cu:152-177, file_reader.cu:127-128); those frames are not rendered
+    // here, so the output keeps its initial zeros until the first pair arrives.
+    bool have_pair = false;
     auto read_next = [&]() {
+        std::fill(pair_set.begin(), pair_set.end(), 0);
+        have_pair = true;
         fwd_slot = 1 - fwd_slot;
         if (!synthetic && next_key >= n_keyframes) return;  // end of the recording: swap only (:169-172)
         for (int e = 0; e < n_eyes; e++) {
@@ -396,35 +457,39 @@ int main(int argc, char **argv) {
         next_key++;
     };
 
-    hipEvent_t ev0, ev1;
-    HIP_OK(hipEventCreate(&ev0));
-    HIP_OK(hipEventCreate(&ev1));
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (!g_dry_run) {
+        HIP_OK(hipEventCreate(&ev0));
+        HIP_OK(hipEventCreate(&ev1));
+    }
     std::vector<float> frame_ms;
     std::vector<uint8_t> host_img((size_t)W * H * 4);
-    // a renderer is handed the keyframe pair once per pair (its planes are
-    // rewritten in place by read_next), so the pair's flow toward the other
-    // keyframe is computed by its first render and reused by the others
-    std::vector<char> pair_set((size_t)n_eyes, 0);
     for (int i = 0; i < total; i += fpa) {          // main.cu:149
         for (int b = 0; b < fpa; b++) {              // main.cu:152
-            if (latency == b) {
-                read_next();
-                std::fill(pair_set.begin(), pair_set.end(), 0);
-            }
+            if (latency == b) read_next();
             refresh_fd();
             const float t = float(latency > b ? fpa + b : b) / float(fpa);  // main.cu:157
             const int frame_number = i + b + 1;
             if (frame_number > total) break;
             float ms_total = 0.f;
+            bool rendered = false;
             for (int e = 0; e < n_eyes; e++) {
                 Eye &E = eyes[e];
-                if (holes_black) HIP_OK(hipMemset(E.out, 0, (size_t)W * H * 4));
-                if (!(b < latency && hold_last)) {
+                if (holes_black && !g_dry_run) HIP_OK(hipMemset(E.out, 0, (size_t)W * H * 4));
+                if (g_dry_run && !(have_pair && !(b < latency && hold_last)))
+                    std::printf("frame %d eye %d skip\n", frame_number - 1, e);
+                if (have_pair && !(b < latency && hold_last)) {
                     isti::RenderInfo ri;
                     ri.resolution[0] = W;
                     ri.resolution[1] = H;
                     ri.alpha = t;
-                    if (synthetic) {  // client camera at absolute client-frame time
+                    if (!client_frames.empty()) {  // main.cu:158-161 with -c
+                        const isti_io_frame &cf = client_frames[(size_t)(i + b)];
+                        if (isti_io_client_camera(cf.w2c, cf.proj, ri.viewproj_mat, ri.inv_viewproj_mat) != ISTI_OK) {
+                            std::fprintf(stderr, "%s\n", isti_io_last_error());
+                            return 2;
+                        }
+                    } else if (synthetic) {  // client camera at absolute client-frame time
                         uint32_t w, h;
                         isti_scene_camera(&cfg, (double)(i + b), view_of(e, 0), ri.viewproj_mat, ri.inv_viewproj_mat, &w, &h);
                     } else {  // getIntermediateViewMatrix (main.cu:26-29, 158-161)
@@ -436,6 +501,14 @@ int main(int argc, char **argv) {
                     }
                     ri.additional_info = {fpa, sfpa, no_shadows, (isti::RenderMode)mode};
                     ri.max_search_iterations = iters;
+                    if (g_dry_run) {
+                        std::printf("frame %d eye %d pair %d t %.9g set %d vp", frame_number - 1, e, next_key - 2,
+                                    (double)t, pair_set[(size_t)e] ? 0 : 1);
+                        for (int k = 0; k < 16; k++) std::printf(" %.9g", (double)ri.viewproj_mat[k]);
+                        std::printf("\n");
+                        pair_set[(size_t)e] = 1;
+                        continue;
+                    }
                     isti::RenderBuffer rb{E.out, (size_t)W * 4, W, H};
                     HIP_OK(hipEventRecord(ev0, nullptr));
                     try {
@@ -453,8 +526,9 @@ int main(int argc, char **argv) {
                     float ms = 0.f;
                     HIP_OK(hipEventElapsedTime(&ms, ev0, ev1));
                     ms_total += ms;
+                    rendered = true;
                 }
-                if (write_images) {
+                if (write_images && !g_dry_run) {
                     HIP_OK(hipMemcpy(host_img.data(), E.out, host_img.size(), hipMemcpyDeviceToHost));
                     char name[64];
                     if (n_eyes == 2) std::snprintf(name, sizeof(name), "%04d_%s.png", frame_number - 1, e == 0 ? "left" : "right");
@@ -462,9 +536,11 @@ int main(int argc, char **argv) {
                     write_png(out_dir / name, host_img.data(), W, H);
                 }
             }
-            frame_ms.push_back(ms_total);
-            std::printf("Frame: %3d / %3d  (%.3f ms)\r", frame_number, total, ms_total);
-            std::fflush(stdout);
+            if (rendered) frame_ms.push_back(ms_total);
+            if (!g_dry_run) {
+                std::printf("Frame: %3d / %3d  (%.3f ms)\r", frame_number, total, ms_total);
+                std::fflush(stdout);
+            }
             if (latency > b && b == fpa - 1) read_next();  // main.cu:188-189
         }
     }

@@ -16,7 +16,7 @@ import ctypes
 import json
 import struct
 import zlib
-from ctypes import POINTER, c_char_p, c_float, c_int32, c_uint8, c_uint32, c_void_p
+from ctypes import POINTER, c_char_p, c_float, c_int32, c_size_t, c_uint8, c_uint32, c_void_p
 from pathlib import Path
 
 import numpy as np
@@ -57,12 +57,15 @@ def io_lib() -> ctypes.CDLL:
         L.isti_io_read_view.argtypes = [c_char_p, c_int32, c_int32, POINTER(View)]
         L.isti_io_free_view.argtypes = [POINTER(View)]
         L.isti_io_free_view.restype = None
-        L.isti_io_decode_png.argtypes = [c_char_p, POINTER(c_uint32), POINTER(c_uint32), c_void_p]
-        L.isti_io_decode_exr.argtypes = [c_char_p, c_char_p, POINTER(c_uint32), POINTER(c_uint32), c_void_p]
+        L.isti_io_decode_png.argtypes = [c_char_p, POINTER(c_uint32), POINTER(c_uint32), c_void_p, c_size_t]
+        L.isti_io_decode_exr.argtypes = [c_char_p, c_char_p, POINTER(c_uint32), POINTER(c_uint32), c_void_p,
+                                         c_size_t]
         for f in ("isti_io_read_camera", "isti_io_read_view", "isti_io_decode_png", "isti_io_decode_exr"):
             getattr(L, f).restype = c_int32
         L.isti_io_camera_at.argtypes = [c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p, c_void_p]
         L.isti_io_camera_at.restype = c_int32
+        L.isti_io_client_camera.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p]
+        L.isti_io_client_camera.restype = c_int32
         L.isti_io_last_error.argtypes = []
         L.isti_io_last_error.restype = c_char_p
         _io = L
@@ -135,12 +138,22 @@ def camera_at(view_fwd, view_bwd, proj_fwd, t: float) -> tuple:
     return v, vp, ivp
 
 
+def client_camera(w2c, proj) -> tuple:
+    """(viewproj, inv_viewproj) of a camera-path frame: normalizeProjMat(proj)
+    * w2c (src/main.cu:158-161, src/file_reader.cu:219-230)."""
+    f = lambda m: np.ascontiguousarray(m, np.float32).reshape(16)
+    a, p = f(w2c), f(proj)
+    vp, ivp = np.zeros(16, np.float32), np.zeros(16, np.float32)
+    _check(io_lib().isti_io_client_camera(a.ctypes.data, p.ctypes.data, vp.ctypes.data, ivp.ctypes.data))
+    return vp, ivp
+
+
 def decode_png(path) -> np.ndarray:
     L = io_lib()
     w, h = c_uint32(), c_uint32()
-    _check(L.isti_io_decode_png(str(path).encode(), ctypes.byref(w), ctypes.byref(h), None))
+    _check(L.isti_io_decode_png(str(path).encode(), ctypes.byref(w), ctypes.byref(h), None, 0))
     out = np.zeros((h.value, w.value, 4), np.uint8)
-    _check(L.isti_io_decode_png(str(path).encode(), ctypes.byref(w), ctypes.byref(h), out.ctypes.data))
+    _check(L.isti_io_decode_png(str(path).encode(), ctypes.byref(w), ctypes.byref(h), out.ctypes.data, out.nbytes))
     return out
 
 
@@ -148,9 +161,10 @@ def decode_exr(path, channels=("R", "G", "B", "A")) -> np.ndarray:
     L = io_lib()
     w, h = c_uint32(), c_uint32()
     names = ",".join(channels).encode()
-    _check(L.isti_io_decode_exr(str(path).encode(), names, ctypes.byref(w), ctypes.byref(h), None))
+    _check(L.isti_io_decode_exr(str(path).encode(), names, ctypes.byref(w), ctypes.byref(h), None, 0))
     out = np.zeros((h.value, w.value, len(channels)), np.float32)
-    _check(L.isti_io_decode_exr(str(path).encode(), names, ctypes.byref(w), ctypes.byref(h), out.ctypes.data))
+    _check(L.isti_io_decode_exr(str(path).encode(), names, ctypes.byref(w), ctypes.byref(h), out.ctypes.data,
+                                out.size))
     return out
 
 
@@ -301,5 +315,21 @@ def write_recording(out_dir, cfg, n_keyframes: int, view: int = 0, cam_name: str
     cam = {"resolution": {"x": w, "y": h}, "clientFps": client_fps, "serverFps": client_fps // spacing,
            "shadowFps": client_fps // spacing * max(cfg.shadow_frames, 1) if cfg.shadows else client_fps,
            "objectMotion": True, "shadowGradients": bool(shadows and cfg.shadows), "frames": frames}
+    (d / "cam.json").write_text(json.dumps(cam, indent=1))
+    return d
+
+
+def write_camera_path(cam_dir, cfg, n_frames: int, client_fps: int, frame_time: float = 1.0, view: int = 0) -> Path:
+    """Writes a client camera path (<cam_dir>/cam.json, the file splinter -c
+    reads; src/file_reader.cu:78-107): frame n is the scene's camera at scene
+    time n * frame_time (the scene's own client-frame units)."""
+    d = Path(cam_dir)
+    d.mkdir(parents=True, exist_ok=True)
+    w, h = S.view_size(cfg, view)
+    frames = []
+    for n in range(n_frames):
+        vm, pm = S.camera_parts(cfg, n * frame_time, view)
+        frames.append({"idx": n, "w2c": _mat_json(vm), "proj": _mat_json(raw_projection(pm))})
+    cam = {"resolution": {"x": w, "y": h}, "clientFps": client_fps, "frames": frames}
     (d / "cam.json").write_text(json.dumps(cam, indent=1))
     return d

@@ -12,7 +12,7 @@
  * Conventions (fixed by the reference, restated here so this header stands on
  * its own):
  *   - matrices are 4x4 float, column-major, element (row r, col c) at m[c*4+r]
- *     (glm::mat4 layout, include/data.h:114-117);
+ *     (glm::mat4 layout of the FrameImage matrices, include/data.h:22-25);
  *   - pixel (x, y) maps to NDC (2(x+.5)/W-1, 2(y+.5)/H-1)
  *     (include/util/helper_common.h:66-75);
  *   - depth planes hold OpenGL NDC z in [-1, 1] (src/file_reader.cu:17-37);
@@ -73,8 +73,8 @@ typedef enum isti_render_mode {
     ISTI_MODE_TIMEWARP = 4
 } isti_render_mode;
 
-/* Keyframe slots: FrameData::fwd / FrameData::bwd (include/data.h:179-180),
- * and for aux cameras AuxCamData::curr / ::next (include/data.h:167). */
+/* Keyframe slots: FrameData::fwd / FrameData::bwd (include/data.h:87-88),
+ * and for aux cameras AuxCamData::curr / ::next (include/data.h:75). */
 typedef enum isti_slot {
     ISTI_SLOT_FWD = 0,
     ISTI_SLOT_BWD = 1

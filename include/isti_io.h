@@ -85,12 +85,23 @@ void isti_io_free_view(isti_io_view *v);
 isti_status isti_io_camera_at(const float view_fwd[16], const float view_bwd[16], const float proj_fwd[16], float t,
                               float view_t[16], float viewproj[16], float inv_viewproj[16]);
 
+/* The client camera of a camera-path file at one client frame
+ * (src/main.cu:158-161 with -c): viewproj = normalizeProjMat(proj) * w2c
+ * (file_reader.cu:219-230) and its inverse, from a frame's raw matrices as
+ * isti_io_read_camera returns them. Either output may be NULL. */
+isti_status isti_io_client_camera(const float w2c[16], const float proj[16], float viewproj[16],
+                                  float inv_viewproj[16]);
+
 /* Image decoders (exposed for tests and tools). PNG -> RGBA8; EXR -> the
  * listed channels (names, comma separated, e.g. "R,G,B"; a missing channel
  * reads 0) as interleaved float. *w and *h receive the size; out may be NULL
- * to query it, else it must hold w*h*4 bytes / w*h*n_channels floats. */
-isti_status isti_io_decode_png(const char *path, uint32_t *w, uint32_t *h, uint8_t *out);
-isti_status isti_io_decode_exr(const char *path, const char *channels, uint32_t *w, uint32_t *h, float *out);
+ * to query it. Otherwise out holds out_bytes bytes / out_floats floats, and a
+ * decoded image larger than that (w*h*4 bytes, w*h*n_channels floats; the
+ * file may change between the query and the fill) is an error that writes
+ * nothing. Images of ISTI_MAX_VIEW_PIXELS pixels or more are rejected. */
+isti_status isti_io_decode_png(const char *path, uint32_t *w, uint32_t *h, uint8_t *out, size_t out_bytes);
+isti_status isti_io_decode_exr(const char *path, const char *channels, uint32_t *w, uint32_t *h, float *out,
+                               size_t out_floats);
 
 /* Message of the calling thread's last failure. Never NULL. */
 const char *isti_io_last_error(void);

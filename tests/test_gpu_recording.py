@@ -2,7 +2,11 @@
 format: a seeded synthetic scene exported with recording.write_recording is
 rendered by splinter (reader, keyframe schedule, interpolated client camera,
 hole retention across frames) and frame by frame through the Python
-Renderer from the same files; the written PNGs must match byte for byte."""
+Renderer from the same files; the written PNGs must match byte for byte.
+The latency and camera-file cases follow the reference's schedule
+(src/main.cu:149-191) in the Python loop: read the next pair when
+latency == b, alpha = (fpa + b) / fpa while latency > b, read again after the
+last frame of a round; -c renders frame n with the file's frames[n]."""
 import subprocess
 from pathlib import Path
 
@@ -52,3 +56,89 @@ def test_splinter_on_a_recording_matches_the_python_renderer(tmp_path, mode):
             exp = buf.cpu().numpy()
             assert np.array_equal(got, exp), (pair, b, int((got != exp).any(-1).sum()))
     r.close()
+
+
+def _python_schedule(rec, cfg, fpa, sfpa, total, latency, camera):
+    """The frames splinter writes, rendered through the Python Renderer with
+    a fresh FrameData per pair (frames before the first pair stay zero)."""
+    r = R.Renderer(0)
+    buf = torch.zeros((cfg.height, cfg.width, 4), dtype=torch.uint8, device="cuda")
+    reads, pair_views, out = 0, {}, []
+    for i in range(0, total, fpa):
+        for b in range(fpa):
+            if latency == b:
+                reads += 1
+            t = float(np.float32(fpa + b if latency > b else b) / np.float32(fpa))
+            n = i + b
+            if n + 1 > total:
+                break
+            if reads:
+                pair = reads - 1
+                if pair not in pair_views:
+                    f, bw = RC.read_view(rec / "main", pair, 0), RC.read_view(rec / "main", pair, 1)
+                    pair_views[pair] = (f, bw, R.FrameData(R.KeyframeView.from_numpy(f), R.KeyframeView.from_numpy(bw)))
+                f, bw, fd = pair_views[pair]
+                vp, ivp = camera(n, f, bw, t)
+                ri = R.RenderInfo((cfg.width, cfg.height), t, vp, ivp, R.AdditionalRenderInfo(fpa, sfpa, False))
+                r.render(buf, ri, fd)
+                torch.cuda.synchronize()
+            out.append(buf.cpu().numpy())
+            if latency > b and b == fpa - 1:
+                reads += 1
+    r.close()
+    return out
+
+
+@pytest.mark.parametrize("latency", [1, "fpa"])
+def test_splinter_latency_schedule_matches_the_python_renderer(tmp_path, latency):
+    """-l 1 and -l fpa over three keyframe pairs: with -l fpa every pair is
+    read by the fallback read after a round (main.cu:188-189), the case where a
+    renderer must still be handed each new pair."""
+    cfg = S.defaults(1, 22)
+    cfg.width, cfg.height = 256, 144
+    rec = tmp_path / "rec"
+    RC.write_recording(rec, cfg, 5)
+    info, _ = RC.read_camera(rec / "main")
+    fpa = 120 // info["server_fps"]
+    sfpa = info["shadow_fps"] // info["server_fps"]
+    lat = fpa if latency == "fpa" else latency
+    total = 3 * fpa + (1 if lat < fpa else fpa)
+    out = tmp_path / "out"
+    print(f"case: splinter -l {lat} fpa {fpa} frames {total}", flush=True)
+    res = subprocess.run([str(SPLINTER), str(rec), str(out), "-w", "-l", str(lat), "--frames", str(total)],
+                         capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    exp = _python_schedule(rec, cfg, fpa, sfpa, total, lat,
+                           lambda n, f, b, t: RC.camera_at(f["view"], b["view"], f["proj"], t)[1:])
+    assert len(exp) == total
+    for n in range(total):
+        got = RC.decode_png(out / f"{n:04d}.png")
+        assert np.array_equal(got, exp[n]), (n, int((got != exp[n]).any(-1).sum()))
+
+
+def test_splinter_client_camera_file_matches_the_python_renderer(tmp_path):
+    """-c: frame n renders with the camera file's frames[n] (normalised
+    projection times w2c); the file's clientFps (60, against the recording's
+    15 Hz server) gives frames_per_anchor 4 and its length the frame count."""
+    cfg = S.defaults(1, 23)
+    cfg.width, cfg.height = 256, 144
+    rec = tmp_path / "rec"
+    RC.write_recording(rec, cfg, 4)
+    info, _ = RC.read_camera(rec / "main")
+    client_fps = 60
+    fpa = client_fps // info["server_fps"]
+    sfpa = info["shadow_fps"] // info["server_fps"]
+    n_frames = 2 * fpa + 3
+    cam_dir = RC.write_camera_path(tmp_path / "client", cfg, n_frames, client_fps, frame_time=1.0)
+    _, frames = RC.read_camera(cam_dir)
+    out = tmp_path / "out"
+    print(f"case: splinter -c fpa {fpa} frames {n_frames}", flush=True)
+    res = subprocess.run([str(SPLINTER), str(rec), str(out), "-w", "-c", str(cam_dir)], capture_output=True,
+                         text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    assert sorted(p.name for p in out.glob("*.png")) == [f"{i:04d}.png" for i in range(n_frames)]
+    exp = _python_schedule(rec, cfg, fpa, sfpa, n_frames, 0,
+                           lambda n, f, b, t: RC.client_camera(frames[n][1], frames[n][2]))
+    for n in range(n_frames):
+        got = RC.decode_png(out / f"{n:04d}.png")
+        assert np.array_equal(got, exp[n]), (n, int((got != exp[n]).any(-1).sum()))

@@ -167,3 +167,79 @@ def test_camera_interpolation_properties():
     Pm = np.diag([2.0, -1.0, 0.5, 1.0]).astype(np.float32).T.reshape(16)
     _, vpp, _ = RC.camera_at(a, b, Pm, 0.5)
     assert np.allclose(vpp.reshape(4, 4).T, np.diag([2.0, -1.0, 0.5, 1.0]) @ vh.reshape(4, 4).T, atol=1e-6)
+
+
+def _exr_header_end(b: bytes) -> int:
+    """Offset just past the attribute list of a single-part EXR (its offset table starts there)."""
+    import struct
+
+    p = 8
+    while True:
+        e = b.index(b"\0", p)
+        if e == p:
+            return p + 1
+        e2 = b.index(b"\0", e + 1)
+        (size,) = struct.unpack_from("<i", b, e2 + 1)
+        p = e2 + 5 + size
+
+
+@pytest.mark.parametrize("offset", [2 ** 64 - 4, 2 ** 64 - 8, 2 ** 63, 10 ** 9])
+def test_exr_rejects_out_of_file_chunk_offsets(tmp_path, offset):
+    """Chunk offsets come from the file: values near 2^64 (where offset + 8
+    wraps) and values past the end are errors, not reads."""
+    import struct
+
+    img = {"R": np.arange(12, dtype=np.float32).reshape(3, 4)}
+    b = bytearray(RC.encode_exr(img, "none", "float"))
+    struct.pack_into("<Q", b, _exr_header_end(bytes(b)), offset)
+    p = tmp_path / "bad.exr"
+    p.write_bytes(bytes(b))
+    with pytest.raises(IOError, match="chunk"):
+        RC.decode_exr(p, ("R",))
+
+
+def test_exr_rejects_oversized_chunk_length(tmp_path):
+    import struct
+
+    b = bytearray(RC.encode_exr({"R": np.ones((3, 4), np.float32)}, "none", "float"))
+    (off,) = struct.unpack_from("<Q", b, _exr_header_end(bytes(b)))
+    struct.pack_into("<i", b, off + 4, 2 ** 31 - 1)
+    p = tmp_path / "bad.exr"
+    p.write_bytes(bytes(b))
+    with pytest.raises(IOError, match="chunk"):
+        RC.decode_exr(p, ("R",))
+
+
+def test_exr_rejects_empty_compression_attribute(tmp_path):
+    """A compression attribute of size 0 as the file's last bytes: no read past the end."""
+    import struct
+
+    head = b"\x76\x2f\x31\x01" + struct.pack("<I", 2)
+    head += b"channels\0chlist\0" + struct.pack("<i", 19) + b"R\0" + struct.pack("<iiii", 2, 0, 1, 1) + b"\0"
+    head += b"compression\0compression\0" + struct.pack("<i", 0)
+    p = tmp_path / "trunc.exr"
+    p.write_bytes(head)
+    with pytest.raises(IOError):
+        RC.decode_exr(p, ("R",))
+
+
+def test_decode_refuses_an_output_buffer_too_small(tmp_path):
+    """The decode ABI takes the output's capacity; a file that grew between the
+    size query and the fill is an error that writes nothing."""
+    import ctypes
+
+    L = RC.io_lib()
+    p = tmp_path / "x.png"
+    p.write_bytes(RC.encode_png(np.zeros((5, 7, 4), np.uint8)))
+    w, h = ctypes.c_uint32(), ctypes.c_uint32()
+    buf = np.full(5 * 7 * 4 - 1, 99, np.uint8)
+    assert L.isti_io_decode_png(str(p).encode(), ctypes.byref(w), ctypes.byref(h), buf.ctypes.data, buf.nbytes) != 0
+    assert (buf == 99).all()
+    q = tmp_path / "x.exr"
+    q.write_bytes(RC.encode_exr({"R": np.ones((3, 4), np.float32)}, "zip", "float"))
+    fb = np.full(11, 7.0, np.float32)
+    assert L.isti_io_decode_exr(str(q).encode(), b"R", ctypes.byref(w), ctypes.byref(h), fb.ctypes.data, fb.size) != 0
+    assert (fb == 7.0).all()
+    fb = np.zeros(12, np.float32)
+    assert L.isti_io_decode_exr(str(q).encode(), b"R", ctypes.byref(w), ctypes.byref(h), fb.ctypes.data, fb.size) == 0
+    assert (fb == 1.0).all()
