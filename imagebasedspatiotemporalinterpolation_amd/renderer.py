@@ -175,8 +175,12 @@ class Renderer:
         for i, a in enumerate(frame_data.aux_cam_images):
             views += [(0, i + 1, a.curr, a.priority), (1, i + 1, a.next, a.priority)]
         for slot, view, kv, prio in views:
-            d = kv.desc(prio)
-            N.check(L.isti_keyframe_set(self._h, slot, view, ctypes.byref(d)), self._h)
+            self.set_view(slot, view, kv, prio)
+
+    def set_view(self, slot: int, view: int, kv: KeyframeView, priority: int = 0) -> None:
+        """Sets one view (0 main, 1.. aux) of one keyframe slot (0 fwd, 1 bwd)."""
+        d = kv.desc(priority)
+        N.check(N.lib().isti_keyframe_set(self._h, int(slot), int(view), ctypes.byref(d)), self._h)
 
     def render(self, render_buffer, render_info: RenderInfo, frame_data: Optional[FrameData] = None,
                stream=None) -> None:

@@ -2,10 +2,17 @@
 
 The reference renders one client frame per call with a host sync inside
 (src/main.cu:149-191, src/renderer.cu:650); a server for many clients needs
-more: this module keeps every session's keyframes and render state resident
+more: this module keeps keyframes and every session's render state resident
 on the GPU, admits sessions by their HBM footprint, and serves interpolation
 requests with continuous batching:
 
+  - keyframes live in a device-resident store per session class (colour, z
+    and the motion toward both neighbouring keyframes of every keyframe);
+    sessions point their renderers at a pair of it, and advance to the next
+    pair as the reference's FrameData::swap + readNextFrame do
+    (include/data.h:110-116, src/file_reader.cu:167-217): the renderer's
+    descriptors are swapped, then both slots are set to the new pair (no
+    plane is copied);
   - each of `n_streams` HIP streams runs one batch at a time
     (isti_render_batch: one persistent launch per stage for the whole batch,
     replayed from a hipGraph);
@@ -32,15 +39,22 @@ import numpy as np
 from . import renderer as R
 from . import scene
 
-# Bytes per pixel and keyframe slot of a session's device state: keyframe
-# planes (colour 4 + z 4 + motion 12) and renderer scratch (splat 8 + cam_ff
-# 16 + full_ff 16), plus the 4-byte output per pixel.
-BYTES_PER_PX_SLOT = 20 + 40
+# Device bytes per pixel and eye. A session's renderer holds, per keyframe
+# slot, the splat cells (8), the cam_ff records (16) and the full_ff records
+# (12), and the session owns its 4-byte output. The class's keyframe store
+# holds per keyframe colour (4), z (4) and the motion toward the next and the
+# previous keyframe (12 each).
+BYTES_PER_PX_SLOT = 8 + 16 + 12
 BYTES_PER_PX_OUT = 4
+BYTES_PER_PX_KEYFRAME = 4 + 4 + 12 + 12
 
 
 def session_footprint(width: int, height: int, eyes: int = 1) -> int:
     return eyes * width * height * (2 * BYTES_PER_PX_SLOT + BYTES_PER_PX_OUT)
+
+
+def store_footprint(width: int, height: int, eyes: int, n_keyframes: int) -> int:
+    return eyes * width * height * BYTES_PER_PX_KEYFRAME * n_keyframes
 
 
 @dataclass
@@ -62,54 +76,103 @@ SERVING_MIX = [  # resolutions and weights of the serving workload
 ]
 
 
-class KeyframeCache:
-    """One generated keyframe pair per session class (host arrays); sessions
-    get their own device copies."""
+class KeyframeStore:
+    """Device-resident keyframes 0..n_keyframes-1 of every eye of each session
+    class, generated on first use. Keyframe k carries its colour and z once
+    and two motion planes: toward k+1 (its role as the fwd keyframe of pair k)
+    and toward k-1 (as the bwd keyframe of pair k-1). Sessions of a class read
+    them without copying."""
 
-    def __init__(self, seed: int = 1):
+    def __init__(self, seed: int = 1, n_keyframes: int = 3):
+        if n_keyframes < 2:
+            raise ValueError("a store needs at least one keyframe pair")
         self.seed = seed
-        self._pairs: Dict[str, tuple] = {}
+        self.n_keyframes = n_keyframes
+        self._kf: Dict[tuple, dict] = {}
+        self._views: Dict[tuple, R.KeyframeView] = {}
+
+    @property
+    def n_pairs(self) -> int:
+        return self.n_keyframes - 1
 
     def config(self, cls: SessionClass):
         return scene.defaults(cls.workload, self.seed, width=cls.width, height=cls.height, n_aux=0,
                               shadows=0, shadow_frames=0)
 
-    def pair(self, cls: SessionClass):
-        if cls.name not in self._pairs:
+    def footprint(self, cls: SessionClass) -> int:
+        return store_footprint(cls.width, cls.height, cls.eyes, self.n_keyframes)
+
+    def _view(self, cls: SessionClass, k: int, eye: int, direction: int) -> R.KeyframeView:
+        import torch
+
+        key = (cls.name, k, eye, direction)
+        if key not in self._views:
             cfg = self.config(cls)
-            eyes = []
+            host = scene.keyframe(cfg, k, eye, direction)
+            base = self._kf.get((cls.name, k, eye))
+            if base is None:
+                base = {"color": torch.from_numpy(host["color"]).cuda(), "z": torch.from_numpy(host["z"]).cuda()}
+                self._kf[(cls.name, k, eye)] = base
+            motion = torch.from_numpy(np.ascontiguousarray(host["motion"])).cuda()
+            self._views[key] = R.KeyframeView(color=base["color"], z=base["z"], viewproj=host["viewproj"],
+                                              inv_viewproj=host["inv_viewproj"], motion=motion)
+        return self._views[key]
+
+    def pair(self, cls: SessionClass, pair: int, eye: int) -> R.FrameData:
+        """FrameData of keyframe pair (pair, pair + 1) of one eye."""
+        if not 0 <= pair < self.n_pairs:
+            raise IndexError(f"pair {pair} outside the store's {self.n_pairs} pairs")
+        return R.FrameData(self._view(cls, pair, eye, +1), self._view(cls, pair + 1, eye, -1))
+
+    def preload(self, cls: SessionClass) -> None:
+        for p in range(self.n_pairs):
             for e in range(cls.eyes):
-                eyes.append((scene.keyframe(cfg, 0, e, +1), scene.keyframe(cfg, 1, e, -1)))
-            self._pairs[cls.name] = (cfg, eyes)
-        return self._pairs[cls.name]
+                self.pair(cls, p, e)
 
 
 class Session:
-    """Device-resident keyframes, renderers (one per eye) and outputs of one client."""
+    """Renderers (one per eye), the current keyframe pair and outputs of one client."""
 
-    def __init__(self, sid: int, cls: SessionClass, cache: KeyframeCache):
+    def __init__(self, sid: int, cls: SessionClass, store: KeyframeStore, pair: int = 0):
         import torch
 
-        self.sid, self.cls = sid, cls
-        self.cfg, eyes = cache.pair(cls)
+        self.sid, self.cls, self.store = sid, cls, store
+        self.cfg = store.config(cls)
         self.fpa = self.cfg.keyframe_spacing
+        self.pair = pair
         self.renderers: List[R.Renderer] = []
-        self.frames: List[R.FrameData] = []
         self.outs = []
-        for fwd, bwd in eyes:
+        for e in range(cls.eyes):
             r = R.Renderer(n_aux=0)
-            fd = R.FrameData(R.KeyframeView.from_numpy(fwd), R.KeyframeView.from_numpy(bwd))
-            r.set_frame(fd)
+            r.set_frame(store.pair(cls, pair, e))
             self.renderers.append(r)
-            self.frames.append(fd)
             self.outs.append(torch.zeros((cls.height, cls.width, 4), dtype=torch.uint8, device="cuda"))
         self.footprint = session_footprint(cls.width, cls.height, cls.eyes)
         self.cameras = {}
+        self.advances = 0
 
-    def info(self, b: int, eye: int, max_iter: int) -> R.RenderInfo:
-        key = (b, eye)
+    def advance(self, pair: int) -> None:
+        """Moves every eye to keyframe pair `pair`. The next pair is reached as
+        the reference does: swap the keyframes (the bwd one becomes fwd), then
+        set the fwd slot's forward motion and the new bwd keyframe; any other
+        pair is set directly. Either way the pair's flows are computed by the
+        first render that follows."""
+        if pair == self.pair:
+            return
+        for e, r in enumerate(self.renderers):
+            fd = self.store.pair(self.cls, pair, e)
+            if pair == self.pair + 1:
+                r.swap()
+            r.set_view(0, 0, fd.fwd)
+            r.set_view(1, 0, fd.bwd)
+        self.pair = pair
+        self.advances += 1
+
+    def info(self, pair: int, b: int, eye: int, max_iter: int) -> R.RenderInfo:
+        """Client frame b of keyframe pair `pair` (alpha = b / fpa)."""
+        key = (pair, b, eye)
         if key not in self.cameras:
-            vp, ivp, _, _ = scene.camera(self.cfg, b, eye)
+            vp, ivp, _, _ = scene.camera(self.cfg, pair * self.fpa + b, eye)
             self.cameras[key] = (vp, ivp)
         vp, ivp = self.cameras[key]
         return R.RenderInfo((self.cls.width, self.cls.height), b / self.fpa, vp, ivp,
@@ -118,7 +181,7 @@ class Session:
     def close(self):
         for r in self.renderers:
             r.close()
-        self.renderers, self.frames, self.outs = [], [], []
+        self.renderers, self.outs = [], []
 
 
 @dataclass
@@ -126,6 +189,7 @@ class Request:
     sid: int
     b: int                     # client frame within the keyframe pair (alpha = b / fpa)
     arrival: float
+    pair: int = 0              # keyframe pair of the store it interpolates
     launched: float = 0.0
     done: float = 0.0
     outs: Optional[list] = None  # per-request output buffers (default: the session's)
@@ -155,18 +219,23 @@ class Stats:
 
 
 class Server:
-    """Admission by HBM footprint + continuous batching over several streams."""
+    """Admission by HBM footprint + continuous batching over several streams.
+
+    The budget covers the sessions' renderer state and outputs and the
+    keyframe store of every class with an admitted session (counted once per
+    class, when its first session is admitted)."""
 
     def __init__(self, hbm_budget: int, max_batch: int = 32, n_streams: int = 2, graph: bool = True,
-                 max_iter: int = -1, cache: Optional[KeyframeCache] = None):
+                 max_iter: int = -1, store: Optional[KeyframeStore] = None):
         import torch
 
         self.budget = hbm_budget
         self.used = 0
         self.max_batch = max_batch
         self.max_iter = max_iter
-        self.cache = cache or KeyframeCache()
+        self.store = store or KeyframeStore()
         self.sessions: Dict[int, Session] = {}
+        self.resident: set = set()  # classes whose keyframe store is counted in `used`
         self.waiting: collections.deque = collections.deque()  # sessions waiting for HBM
         self.pending: collections.deque = collections.deque()
         self.busy: set = set()  # session ids with work on the GPU
@@ -178,25 +247,34 @@ class Server:
         self._next_sid = 0
 
     # -- sessions ---------------------------------------------------------
-    def open_session(self, cls: SessionClass) -> int:
-        """Queues a session; it becomes active once its footprint fits the budget."""
+    def footprint(self, cls: SessionClass) -> int:
+        """Bytes admitting a session of `cls` would add now."""
         fp = session_footprint(cls.width, cls.height, cls.eyes)
-        if fp > self.budget:
-            raise ValueError(f"session {cls.name} needs {fp} B, more than the whole budget")
+        if cls.name not in self.resident:
+            fp += self.store.footprint(cls)
+        return fp
+
+    def open_session(self, cls: SessionClass, pair: int = 0) -> int:
+        """Queues a session starting at keyframe pair `pair`; it becomes active
+        once its footprint fits the budget."""
+        need = session_footprint(cls.width, cls.height, cls.eyes) + self.store.footprint(cls)
+        if need > self.budget:
+            raise ValueError(f"session {cls.name} needs {need} B, more than the whole budget")
         sid = self._next_sid
         self._next_sid += 1
-        self.waiting.append((sid, cls))
+        self.waiting.append((sid, cls, pair))
         self._admit()
         return sid
 
     def _admit(self):
         while self.waiting:
-            sid, cls = self.waiting[0]
-            fp = session_footprint(cls.width, cls.height, cls.eyes)
+            sid, cls, pair = self.waiting[0]
+            fp = self.footprint(cls)
             if self.used + fp > self.budget:
                 return
             self.waiting.popleft()
-            self.sessions[sid] = Session(sid, cls, self.cache)
+            self.sessions[sid] = Session(sid, cls, self.store, pair)
+            self.resident.add(cls.name)
             self.used += fp
 
     def active(self, sid: int) -> bool:
@@ -246,12 +324,10 @@ class Server:
             return
         for r in chosen:
             s = self.sessions[r.sid]
-            if r.b == 1:  # the first frame of a keyframe pair: the pair's flows are computed once more
-                for rend, fd in zip(s.renderers, s.frames):
-                    rend.set_frame(fd)
+            s.advance(r.pair)
             outs = r.outs if r.outs is not None else s.outs
             for e, rend in enumerate(s.renderers):
-                work.append((rend, s.info(r.b, e, self.max_iter), outs[e]))
+                work.append((rend, s.info(r.pair, r.b, e, self.max_iter), outs[e]))
             r.launched = now
             self.busy.add(r.sid)
         import torch
@@ -292,44 +368,60 @@ def poisson_arrivals(rate_hz: float, duration_s: float, rng) -> np.ndarray:
     return t[t < duration_s]
 
 
+def frame_schedule(n: int, fpa: int, phase: int, n_pairs: int) -> tuple:
+    """(pair, b) of a session's n-th request: client frames 1..fpa-1 of a
+    pair, then the next pair (cycling through the store's pairs)."""
+    per_pair = fpa - 1
+    return (phase + n // per_pair) % n_pairs, 1 + n % per_pair
+
+
 def run_mix(n_sessions: int = 48, rate_hz: float = 90.0, duration_s: float = 2.0, seed: int = 1,
             hbm_budget: int = 64 << 30, max_batch: int = 32, n_streams: int = 2, graph: bool = True,
-            max_iter: int = -1, cache: Optional[KeyframeCache] = None) -> dict:
-    """Serving mix: sessions drawn from SERVING_MIX, Poisson arrivals per session."""
+            max_iter: int = -1, store: Optional[KeyframeStore] = None) -> dict:
+    """Serving mix: sessions drawn from SERVING_MIX, Poisson arrivals per
+    session; every session walks through the store's keyframe pairs."""
     rng = np.random.default_rng(seed)
     classes = [c for c, _ in SERVING_MIX]
     weights = np.array([w for _, w in SERVING_MIX])
     picks = rng.choice(len(classes), size=n_sessions, p=weights / weights.sum())
-    srv = Server(hbm_budget, max_batch=max_batch, n_streams=n_streams, graph=graph, max_iter=max_iter, cache=cache)
-    sids = [srv.open_session(classes[k]) for k in picks]
+    store = store or KeyframeStore(seed)
+    for k in sorted(set(picks)):
+        store.preload(classes[k])
+    srv = Server(hbm_budget, max_batch=max_batch, n_streams=n_streams, graph=graph, max_iter=max_iter, store=store)
+    phases = rng.integers(0, store.n_pairs, n_sessions)
+    sids = [srv.open_session(classes[k], int(p)) for k, p in zip(picks, phases)]
     events = []
     for sid in sids:
         for t in poisson_arrivals(rate_hz, duration_s, rng):
             events.append((float(t), sid))
     events.sort()
     # warm up every session once (graph capture, scratch allocation)
-    for sid in sids:
+    for sid, p in zip(sids, phases):
         if srv.active(sid):
-            srv.submit(Request(sid, 1, time.perf_counter()))
+            srv.submit(Request(sid, 1, time.perf_counter(), pair=int(p)))
     srv.drain()
     srv.stats = Stats()
     t0 = time.perf_counter()
     k = 0
-    frame = collections.Counter()
+    count = collections.Counter()
     while k < len(events) or srv.pending or any(f is not None for f in srv.inflight):
         now = time.perf_counter() - t0
         while k < len(events) and events[k][0] <= now:
             t, sid = events[k]
             k += 1
             if srv.active(sid):
-                frame[sid] += 1
-                srv.submit(Request(sid, 1 + frame[sid] % (srv.sessions[sid].fpa - 1), t0 + t))
+                s = srv.sessions[sid]
+                count[sid] += 1
+                pair, b = frame_schedule(count[sid], s.fpa, int(phases[sid]), store.n_pairs)
+                srv.submit(Request(sid, b, t0 + t, pair=pair))
         srv.step()
     out = srv.stats.summary()
+    active = [srv.sessions[s] for s in sids if srv.active(s)]
     out.update({"sessions": n_sessions, "rate_hz_per_session": rate_hz, "duration_s": duration_s,
-                "offered_mpix_per_s": round(sum(srv.sessions[s].cls.width * srv.sessions[s].cls.height *
-                                                srv.sessions[s].cls.eyes for s in sids if srv.active(s))
+                "offered_mpix_per_s": round(sum(s.cls.width * s.cls.height * s.cls.eyes for s in active)
                                             * rate_hz / 1e6, 1),
+                "pair_advances": int(sum(s.advances for s in active)),
+                "hbm_used_bytes": int(srv.used),
                 "streams": n_streams, "graph": graph})
     srv.close()
     return out

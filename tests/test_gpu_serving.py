@@ -1,30 +1,101 @@
-"""Serving loop: admission by HBM footprint, and every request renders the
-same bytes alone and inside the continuous-batching mix."""
+"""Serving loop: admission by HBM footprint, keyframe-pair advances, and every
+request renders the same bytes alone and inside the continuous-batching mix
+- at small sizes and at the serving workload's own resolutions (640x360,
+1280x720, 1920x1080 and stereo 2x2064x2208)."""
 import time
 
 import numpy as np
 import pytest
 import torch
 
+from imagebasedspatiotemporalinterpolation_amd import renderer as R
+from imagebasedspatiotemporalinterpolation_amd import scene
 from imagebasedspatiotemporalinterpolation_amd import serving as S
 
 pytestmark = pytest.mark.gpu
 
 SMALL = [S.SessionClass("320x180", 1, 320, 180), S.SessionClass("480x272", 1, 480, 272),
          S.SessionClass("stereo 258x276", 3, 258, 276, eyes=2)]
+FULL = [c for c, _ in S.SERVING_MIX]
 
 
 @pytest.fixture(scope="module")
-def cache():
-    return S.KeyframeCache(seed=3)
+def store():
+    return S.KeyframeStore(seed=3, n_keyframes=3)
 
 
-def test_admission_by_hbm_budget(cache):
-    fp = S.session_footprint(320, 180)
-    srv = S.Server(hbm_budget=int(fp * 1.5), cache=cache)
-    a = srv.open_session(SMALL[0])
-    b = srv.open_session(SMALL[0])
+def replay_alone(store, cls, reqs, max_iter=-1):
+    """A session's requests rendered one by one by fresh renderers that are
+    handed each pair with set_frame; returns the outputs."""
+    cfg = store.config(cls)
+    fpa = cfg.keyframe_spacing
+    rends = [R.Renderer(0) for _ in range(cls.eyes)]
+    cur, outs = None, []
+    for pair, b, prev in reqs:
+        if pair != cur:
+            for e, r in enumerate(rends):
+                r.set_frame(store.pair(cls, pair, e))
+            cur = pair
+        res = []
+        for e, r in enumerate(rends):
+            vp, ivp, _, _ = scene.camera(cfg, pair * fpa + b, e)
+            info = R.RenderInfo((cls.width, cls.height), b / fpa, vp, ivp,
+                                R.AdditionalRenderInfo(fpa, 0, False, R.RenderMode.OURS), max_iter)
+            alone = torch.from_numpy(prev[e].copy()).cuda()
+            r.render(alone, info)
+            res.append(alone)
+        torch.cuda.synchronize()
+        outs.append([o.cpu().numpy() for o in res])
+    for r in rends:
+        r.close()
+    return outs
+
+
+def run_interleaved(srv, sids, n_each, rng, per_session_pairs):
+    """Interleaved arrivals, n_each requests per session in shuffled order:
+    each session walks through its frames (pairs advance after fpa - 1
+    frames); per-request outputs start from random bytes (holes keep them)."""
+    t0 = time.perf_counter()
+    log = {sid: [] for sid in sids}
+    count = {sid: 0 for sid in sids}
+    order = [sid for sid in sids for _ in range(n_each)]
+    rng.shuffle(order)
+    for k, sid in enumerate(order):
+        s = srv.sessions[sid]
+        pair, b = S.frame_schedule(count[sid], s.fpa, per_session_pairs[sid], srv.store.n_pairs)
+        count[sid] += 1
+        outs = [torch.from_numpy(rng.integers(0, 256, (s.cls.height, s.cls.width, 4), dtype=np.uint8)).cuda()
+                for _ in range(s.cls.eyes)]
+        prev = [o.cpu().numpy() for o in outs]
+        r = S.Request(sid, b, t0, pair=pair, outs=outs)
+        log[sid].append((r, prev))
+        srv.submit(r)
+        if k % 3 == 0:
+            srv.step()
+    srv.drain()
+    return log
+
+
+def check_alone(srv, log):
+    for sid, entries in log.items():
+        if not entries:
+            continue
+        s = srv.sessions[sid]
+        alone = replay_alone(srv.store, s.cls, [(r.pair, r.b, prev) for r, prev in entries], srv.max_iter)
+        for (r, _), a in zip(entries, alone):
+            for e in range(s.cls.eyes):
+                np.testing.assert_array_equal(r.outs[e].cpu().numpy(), a[e],
+                                              err_msg=f"session {sid} ({s.cls.name}) pair {r.pair} frame {r.b} eye {e}")
+
+
+def test_admission_by_hbm_budget(store):
+    cls = SMALL[0]
+    first = S.session_footprint(cls.width, cls.height) + store.footprint(cls)
+    srv = S.Server(hbm_budget=first + S.session_footprint(cls.width, cls.height) // 2, store=store)
+    a = srv.open_session(cls)
+    b = srv.open_session(cls)
     assert srv.active(a) and not srv.active(b)  # the second does not fit yet
+    assert srv.used == first                    # the class's keyframes are counted once, with the first session
     srv.submit(S.Request(a, 1, time.perf_counter()))
     srv.drain()
     srv.close_session(a)
@@ -35,31 +106,43 @@ def test_admission_by_hbm_budget(cache):
 
 
 @pytest.mark.parametrize("graph", [True, False])
-def test_results_same_alone_and_in_mix(cache, graph):
-    srv = S.Server(hbm_budget=1 << 34, max_batch=4, n_streams=2, graph=graph, cache=cache)
-    sids = [srv.open_session(SMALL[i % 3]) for i in range(7)]
+def test_results_same_alone_and_in_mix(store, graph):
+    srv = S.Server(hbm_budget=1 << 34, max_batch=4, n_streams=2, graph=graph, store=store)
+    sids = [srv.open_session(SMALL[i % 3], pair=i % 2) for i in range(7)]
     rng = np.random.default_rng(0)
-    reqs = []
-    t0 = time.perf_counter()
-    for k in range(40):  # interleaved arrivals; several per session, some back to back
-        sid = sids[int(rng.integers(0, len(sids)))]
-        s = srv.sessions[sid]
-        outs = [torch.from_numpy(rng.integers(0, 256, (s.cls.height, s.cls.width, 4), dtype=np.uint8)).cuda()
-                for _ in range(s.cls.eyes)]
-        prev = [o.cpu().numpy() for o in outs]
-        r = S.Request(sid, 1 + k % (s.fpa - 1), t0, outs=outs)
-        reqs.append((r, prev))
-        srv.submit(r)
-        if k % 3 == 0:
-            srv.step()
-    srv.drain()
+    log = run_interleaved(srv, sids, 7, rng, {sid: i % 2 for i, sid in enumerate(sids)})
     assert srv.stats.batches >= 2 and max(srv.stats.batch_sizes) > 1
-    for r, prev in reqs:
-        s = srv.sessions[r.sid]
-        for e, rend in enumerate(s.renderers):
-            alone = torch.from_numpy(prev[e].copy()).cuda()
-            rend.render(alone, s.info(r.b, e, srv.max_iter))
-            torch.cuda.synchronize()
-            np.testing.assert_array_equal(r.outs[e].cpu().numpy(), alone.cpu().numpy(),
-                                          err_msg=f"session {r.sid} frame {r.b} eye {e}")
+    assert sum(srv.sessions[s].advances for s in sids) >= 3
+    check_alone(srv, log)
+    srv.close()
+
+
+def test_serving_mix_at_full_resolutions(store):
+    """The serving workload's resolutions: one session of each class (two of
+    640x360), interleaved arrivals, every session crossing into another
+    keyframe pair (the one starting at pair 0 by swap + set, the others by a
+    direct set); a sixth session waits for HBM until one closes."""
+    t_start = time.perf_counter()
+    for cls in FULL:
+        store.preload(cls)
+    print(f"case: store ready in {time.perf_counter() - t_start:.1f} s", flush=True)
+    classes = [FULL[0], FULL[1], FULL[2], FULL[3], FULL[0]]
+    need = sum(S.session_footprint(c.width, c.height, c.eyes) for c in classes)
+    need += sum(store.footprint(c) for c in FULL)
+    srv = S.Server(hbm_budget=need, max_batch=8, n_streams=3, store=store)
+    starts = [0, 1, 0, 1, 1]
+    sids = [srv.open_session(c, pair=p) for c, p in zip(classes, starts)]
+    assert all(srv.active(s) for s in sids) and srv.used == need
+    late = srv.open_session(FULL[1], pair=0)  # no room left
+    assert not srv.active(late)
+    rng = np.random.default_rng(5)
+    log = run_interleaved(srv, sids, 7, rng, dict(zip(sids, starts)))
+    assert max(srv.stats.batch_sizes) > 1
+    assert all(srv.sessions[s].advances >= 1 for s in sids), [srv.sessions[s].advances for s in sids]
+    print(f"case: {srv.stats.completed} requests in {srv.stats.batches} batches", flush=True)
+    check_alone(srv, log)
+    srv.close_session(sids[1])  # a 1280x720 session leaves: the waiting one fits
+    assert srv.active(late)
+    log2 = run_interleaved(srv, [late], 4, rng, {late: 0})
+    check_alone(srv, log2)
     srv.close()
