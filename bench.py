@@ -32,6 +32,11 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue: a wave64 instruction occupies a 16-lane SIMD for 4 cycles;
+# MI355X has 256 CUs x 4 SIMDs at up to 2.4 GHz.
+SIMDS = 1024
+CLOCK_HZ = 2.4e9
+VALU_PEAK_GINST_S = SIMDS * CLOCK_HZ / 4 / 1e9
 METRIC = "interpolated Mpix/s & p99 frame ms at 1080p, learned-init vs plain search"
 
 
@@ -45,7 +50,8 @@ def parse():
     p.add_argument("--mode", type=int, default=0)
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-baseline sample budget")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--traffic-file", default=str(ROOT / "profiles" / "traffic_1080p.json"))
+    p.add_argument("--counters-file", default=str(ROOT / "profiles" / "counters_1080p.json"),
+                   help="per-kernel HBM traffic and instruction counts of this workload (rocprofv3 summary)")
     p.add_argument("--serving-seconds", type=float, default=2.0,
                    help="duration of the 48-session serving mix (0 disables it)")
     p.add_argument("--stereo-frames", type=int, default=60, help="stereo 2x2064x2208 frames timed (0 disables)")
@@ -86,29 +92,65 @@ def algorithmic_bytes(src_px: int, out_px: int, shadows: bool) -> dict:
     }
 
 
+def cpu_model() -> str | None:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(fwd, bwd, infos, budget_s: float):
+    """The CPU oracle on the same frames: one thread, then all the cores this
+    process may use (its affinity set, capped by OMP_NUM_THREADS when set),
+    each for half the budget, cycling over the pair's intermediate frames."""
     from oracle import oracle as O
 
-    threads = min(16, os.cpu_count() or 1)
+    available = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    all_threads = min(available, int(omp)) if omp.isdigit() and int(omp) > 0 else available
     h, w = fwd[0]["z"].shape
     out = np.zeros((h, w, 4), np.uint8)
     pr = [0] + [3] * (len(fwd) - 1)
-    done, t0 = 0, time.perf_counter()
-    # the intermediate frames of the pair, cycled until the budget is spent
-    while True:
-        O.render(fwd, bwd, infos[done % len(infos)], out, priorities=pr, threads=threads)
-        done += 1
-        if time.perf_counter() - t0 >= budget_s:
-            break
-    dt = time.perf_counter() - t0
+
+    def run(threads, seconds):
+        done, t0 = 0, time.perf_counter()
+        while True:
+            O.render(fwd, bwd, infos[done % len(infos)], out, priorities=pr, threads=threads)
+            done += 1
+            if time.perf_counter() - t0 >= seconds:
+                break
+        dt = time.perf_counter() - t0
+        return done, dt, done * w * h / dt / 1e6
+
+    n1, t1, v1 = run(1, budget_s / 2)
+    na, ta, va = run(all_threads, budget_s / 2)
     return {
-        "value": round(done * w * h / dt / 1e6, 3),
+        "value": round(va, 3),
         "unit": "Mpix/s",
-        "cores": threads,
+        "cores": all_threads,
         "kind": "port",
-        "sample": f"{done} frames cycling over the {len(infos)} intermediate 1920x1080 frames of one keyframe "
-                  f"pair (same inputs as the GPU), oracle with {threads} OpenMP threads, {dt:.1f} s",
+        "sample": f"{na} frames cycling over the {len(infos)} intermediate 1920x1080 frames of one keyframe "
+                  f"pair (same inputs as the GPU), oracle with {all_threads} OpenMP threads, {ta:.1f} s",
+        "single_thread": {"value": round(v1, 4), "unit": "Mpix/s", "frames": n1, "seconds": round(t1, 1)},
+        "nproc": os.cpu_count(),
+        "cpus_available": available,
+        "cpu_model": cpu_model(),
     }
+
+
+def resolve_kernel_name(window: int, always_valid: bool, n_aux: int, tile) -> str:
+    return f"resolve_kernel<{window}, {'true' if always_valid else 'false'}, {n_aux}, {tile[0]}, {tile[1]}>"
+
+
+def load_counters(path: Path) -> dict:
+    try:
+        return json.loads(Path(path).read_text())
+    except (OSError, ValueError):
+        return {}
 
 
 def stereo_bench(n_frames: int, seed: int) -> dict:
@@ -378,13 +420,33 @@ def main():
     stage_avg_ms = {name: totals[i] / max(count, 1) for i, name in enumerate(("clear", "splat", "resolve"))}
     dom = max(stage_avg_ms, key=stage_avg_ms.get)
     achieved = alg[dom] / (stage_avg_ms[dom] * 1e-3) / 1e9
-    traffic = None
-    tf = Path(args.traffic_file)
-    if tf.exists():
-        try:
-            traffic = json.loads(tf.read_text()).get(dom)
-        except (ValueError, OSError):
-            traffic = None
+    # the kernel variant each stage ran, and its counters from the committed
+    # rocprofv3 summary of this workload (same variant, same inputs)
+    tile = tuned["tile"] if tuned else [16, 16]
+    window = 3 if args.mode == 1 else 1
+    variants = {
+        "resolve": resolve_kernel_name(window, args.mode == 1, cfg.n_aux, tile),
+        "splat": "reproject_splat_binned_kernel" if tuned and tuned["splat"] == "lds-binned"
+        else "reproject_splat_kernel",
+        "clear": "clear_splat_kernel",
+    }
+    counters = load_counters(Path(args.counters_file)).get("kernels", {})
+
+    def counter(stage, key):
+        c = counters.get(variants[stage])
+        return None if c is None else c.get(key)
+
+    traffic = counter(dom, "traffic")
+    valu = counter(dom, "valu_insts")
+    valu_issue = None
+    if valu is not None:
+        issue_s = valu * 4 / (SIMDS * CLOCK_HZ)
+        valu_issue = {"insts_per_launch": int(valu),
+                      "achieved_ginst_s": round(valu / (stage_avg_ms[dom] * 1e-3) / 1e9, 1),
+                      "peak_ginst_s": VALU_PEAK_GINST_S,
+                      "issue_us": round(issue_s * 1e6, 1),
+                      "frac": round(issue_s / (stage_avg_ms[dom] * 1e-3), 4)}
+    hbm_frac = achieved / HBM_PEAK_GBS
 
     value = n_frames * out_px / wall / 1e6
     res = {
@@ -420,18 +482,23 @@ def main():
             "source_px_per_launch": src_px,
             "atomic_min_per_s_upper": round(src_px / (stage_avg_ms["splat"] * 1e-3), 1),
             "algorithmic_gbps": round(alg["splat"] / (stage_avg_ms["splat"] * 1e-3) / 1e9, 1),
-            "traffic_gbps": (round(json.loads(tf.read_text())["splat"] / (stage_avg_ms["splat"] * 1e-3) / 1e9, 1)
-                             if tf.exists() else None),
+            "traffic_gbps": (round(counter("splat", "traffic") / (stage_avg_ms["splat"] * 1e-3) / 1e9, 1)
+                             if counter("splat", "traffic") is not None else None),
         },
+        # achieved/peak/frac: algorithmic HBM bytes against the HBM peak; the
+        # VALU-issue fraction beside it; `bound` names the larger of the two
         "roofline": {
             "kernel": dom,
-            "bound": "hbm",
+            "variant": variants[dom],
+            "bound": "valu" if valu_issue and valu_issue["frac"] > hbm_frac else "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "frac": round(hbm_frac, 4),
             "algorithmic_bytes": alg[dom],
             "traffic": traffic,
+            "valu_issue": valu_issue,
+            "counters": Path(args.counters_file).name if counters else None,
         },
     }
     if args.learned_frames > 0:

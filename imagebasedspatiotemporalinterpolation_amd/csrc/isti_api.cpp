@@ -920,10 +920,13 @@ isti_status isti_autotune(isti_renderer *r, const isti_render_info *info, uint8_
     hipEvent_t ev[2] = {nullptr, nullptr};
     e = hipEventCreate(&ev[0]);
     if (e == hipSuccess) e = hipEventCreate(&ev[1]);
-    // splat stage (clear + reprojection/splat) with each splat variant
+    // splat stage (clear + reprojection/splat) with each splat variant. The
+    // first warm-up launch writes the pair's flow toward the other keyframe;
+    // the timed launches then skip that write, as the pair's later frames do.
     float splat_ms[2] = {};
     for (int v = 0; v < 2 && e == hipSuccess; v++) {
         if ((st = splat_stage(r, P, s, v == 1)) != ISTI_OK) break;  // warm-up
+        for (int j = 0; j < P.nj; j++) P.jobs[j].write_full = 0;
         if ((e = hipEventRecord(ev[0], s)) != hipSuccess) break;
         for (int i = 0; i < reps && st == ISTI_OK; i++) st = splat_stage(r, P, s, v == 1);
         if (st != ISTI_OK) break;
