@@ -29,6 +29,7 @@ EXPORTED_SYMBOLS = (
     "isti_keyframe_set",
     "isti_keyframe_swap",
     "isti_render",
+    "isti_render_noalloc",
     "isti_scratch",
     "isti_set_stage_timing",
     "isti_stage_times",
@@ -138,6 +139,8 @@ def lib() -> ctypes.CDLL:
     L.isti_keyframe_swap.restype = c_int32
     L.isti_render.argtypes = [c_void_p, POINTER(RenderInfoC), c_void_p, c_size_t, c_void_p]
     L.isti_render.restype = c_int32
+    L.isti_render_noalloc.argtypes = [c_void_p, POINTER(RenderInfoC), c_void_p, c_size_t, c_void_p]
+    L.isti_render_noalloc.restype = c_int32
     L.isti_scratch.argtypes = [c_void_p, c_int32, c_int32, POINTER(ScratchView)]
     L.isti_scratch.restype = c_int32
     L.isti_set_stage_timing.argtypes = [c_void_p, c_int32]

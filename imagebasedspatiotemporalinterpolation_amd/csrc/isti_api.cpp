@@ -25,6 +25,7 @@
 
 #include "../../include/isti.h"
 #include "../../include/isti_learned.h"
+#include "isti_internal.hpp"
 #include "isti_kernels.hpp"
 
 using namespace isti;
@@ -226,11 +227,15 @@ hipError_t mark(isti_renderer *r, int i, hipStream_t s) {
     return e;
 }
 
+// ext_rows / ext_mean: device scratch for the row sums (height doubles) and
+// the mean, or null for the renderer's own (allocated on first use).
 isti_status render_timewarp(isti_renderer *r, const isti_render_info *info, uint8_t *out, size_t pitch,
-                            hipStream_t s) {
+                            hipStream_t s, double *ext_rows = nullptr, float *ext_mean = nullptr) {
     // src/renderer.cu:615-629
     const isti_view_desc &f = r->views[0][0];
-    if (r->row_sums_cap < f.height) {
+    if (ext_rows) {
+        // caller-provided scratch: nothing is allocated
+    } else if (r->row_sums_cap < f.height) {
         if (r->row_sums) (void)hipFree(r->row_sums);
         r->row_sums = nullptr;
         r->row_sums_cap = 0;
@@ -238,12 +243,14 @@ isti_status render_timewarp(isti_renderer *r, const isti_render_info *info, uint
         if (e != hipSuccess) return dev_fail(r, e, "hipMalloc(row sums)");
         r->row_sums_cap = f.height;
     }
-    if (!r->mean_z) {
+    if (!ext_mean && !r->mean_z) {
         hipError_t e = hipMalloc(&r->mean_z, sizeof(float));
         if (e != hipSuccess) return dev_fail(r, e, "hipMalloc(mean z)");
     }
+    double *rows = ext_rows ? ext_rows : r->row_sums;
+    float *mean = ext_mean ? ext_mean : r->mean_z;
     hipError_t e = mark(r, 0, s);
-    if (e == hipSuccess) e = launch_mean_z(f.z, f.width, f.height, r->row_sums, r->mean_z, s);
+    if (e == hipSuccess) e = launch_mean_z(f.z, f.width, f.height, rows, mean, s);
     if (e != hipSuccess) return dev_fail(r, e, "mean z launch");
     if ((e = mark(r, 1, s)) != hipSuccess || (e = mark(r, 2, s)) != hipSuccess) return dev_fail(r, e, "event");
 
@@ -252,7 +259,7 @@ isti_status render_timewarp(isti_renderer *r, const isti_render_info *info, uint
     a.inv_vp_from = to_mat(info->inv_viewproj);
     a.inv_vp_to = to_mat(f.inv_viewproj);
     a.vp_to = to_mat(f.viewproj);
-    a.depth_to = r->mean_z;
+    a.depth_to = mean;
     a.img_to = reinterpret_cast<const uchar4 *>(f.color);
     a.w_to = f.width;
     a.h_to = f.height;
@@ -289,12 +296,21 @@ struct FramePlan {
     ResolveArgs ra;
     int window = 1;
     bool always_valid = false;
+    bool own_scratch = true;  // false: planes carved from caller-provided scratch
 };
 
+// ext: per (slot, view) scratch planes carved from caller-provided memory, or
+// null for the renderer's own. Carved planes may hold another call's data, so
+// the flow toward the other keyframe is always recomputed.
 isti_status plan_frame(isti_renderer *r, const isti_render_info *info, uint8_t *out, size_t out_pitch,
-                       FramePlan &P) {
+                       FramePlan &P, const Scratch (*ext)[MAX_VIEWS] = nullptr) {
     const int nv = 1 + r->n_aux;
-    for (int sl = 0; sl < 2; sl++)
+    Scratch (*scratch)[MAX_VIEWS] = r->scratch;
+    if (ext) {
+        scratch = const_cast<Scratch (*)[MAX_VIEWS]>(ext);
+        P.own_scratch = false;
+    }
+    for (int sl = 0; sl < 2 && !ext; sl++)
         for (int v = 0; v < nv; v++) {
             const isti_status st = ensure_scratch(r, r->scratch[sl][v], r->views[sl][v].width, r->views[sl][v].height);
             if (st != ISTI_OK) return st;
@@ -309,7 +325,7 @@ isti_status plan_frame(isti_renderer *r, const isti_render_info *info, uint8_t *
         for (int sl = 0; sl < 2; sl++) {
             const isti_view_desc &d = r->views[sl][v];
             const isti_view_desc &other = r->views[1 - sl][v];
-            Scratch &sc = r->scratch[sl][v];
+            Scratch &sc = scratch[sl][v];
             SplatJob &J = P.jobs[P.nj++];
             const bool hm = has_motion(d);
             J.z = d.z;
@@ -326,7 +342,7 @@ isti_status plan_frame(isti_renderer *r, const isti_render_info *info, uint8_t *
             J.main_slot = sl;
             J.inv_w = 1.0 / (double)d.width;
             J.rcp_w = (float)(1.0 / (double)d.width);
-            J.write_full = sc.full_ok ? 0 : 1;
+            J.write_full = (sc.full_ok && !ext) ? 0 : 1;
             J.inv_h = 1.0 / (double)d.height;
             J.inv_vp_from = to_mat(d.inv_viewproj);
             J.vp_to = to_mat(other.viewproj);
@@ -337,8 +353,8 @@ isti_status plan_frame(isti_renderer *r, const isti_render_info *info, uint8_t *
     for (int v = 0; v < nv; v++) {
         const int prio_f = v == 0 ? 0 : r->views[0][v].priority;  // main priority is 0 (renderer.cu:654)
         const int prio_b = v == 0 ? 0 : r->views[1][v].priority;
-        ra.fwd[v] = resolve_view(r->views[0][v], r->scratch[0][v], true, prio_f);
-        ra.bwd[v] = resolve_view(r->views[1][v], r->scratch[1][v], false, prio_b);
+        ra.fwd[v] = resolve_view(r->views[0][v], scratch[0][v], true, prio_f);
+        ra.bwd[v] = resolve_view(r->views[1][v], scratch[1][v], false, prio_b);
         set_ratio(ra.fwd[v], info->width, info->height);
         set_ratio(ra.bwd[v], info->width, info->height);
     }
@@ -426,7 +442,7 @@ isti_status splat_stage(isti_renderer *r, const FramePlan &P, hipStream_t s, boo
     if ((e = launch_clear(ca, s)) != hipSuccess) return dev_fail(r, e, "clear launch");
     if ((e = mark(r, 1, s)) != hipSuccess) return dev_fail(r, e, "event");
     if ((e = launch_splat(sa, binned, s)) != hipSuccess) return dev_fail(r, e, "splat launch");
-    mark_full_written(r);
+    if (P.own_scratch) mark_full_written(r);
     if ((e = mark(r, 2, s)) != hipSuccess) return dev_fail(r, e, "event");
     return ISTI_OK;
 }
@@ -856,20 +872,77 @@ isti_status isti_keyframe_swap(isti_renderer *r) {
     return ISTI_OK;
 }
 
-isti_status isti_render(isti_renderer *r, const isti_render_info *info, uint8_t *out, size_t out_pitch,
-                        void *stream) {
+}  // extern "C"
+
+namespace {
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+}  // namespace
+
+int isti_internal_device(const isti_renderer *r) { return r->device; }
+
+isti_status isti_internal_fail(isti_renderer *r, isti_status st, const std::string &msg) { return fail(r, st, msg); }
+
+// Scratch bytes a frame of `info` needs when its planes are carved from one
+// caller-provided block (each plane 256-byte aligned).
+size_t isti_internal_scratch_bytes(const isti_renderer *r, const isti_render_info *info) {
+    if (info->mode == ISTI_MODE_TIMEWARP) return align256(sizeof(double) * r->views[0][0].height) + 256;
+    size_t bytes = 0;
+    for (int sl = 0; sl < 2; sl++)
+        for (int v = 0; v <= r->n_aux; v++) {
+            const size_t n = (size_t)r->views[sl][v].width * r->views[sl][v].height;
+            bytes += align256(n * sizeof(unsigned long long)) + align256(n * sizeof(float4)) +
+                     align256(n * sizeof(Flow3));
+        }
+    return bytes;
+}
+
+// Renderer::render with the renderer's own scratch (ext_base null) or with
+// every scratch plane carved from ext_bytes of device memory at ext_base, in
+// which case nothing is allocated.
+isti_status isti_internal_render(isti_renderer *r, const isti_render_info *info, uint8_t *out, size_t out_pitch,
+                                 void *stream, void *ext_base, size_t ext_bytes) {
     if (!r) return ISTI_ERR_INVALID_ARG;
     if (!info || !out) return fail(r, ISTI_ERR_INVALID_ARG, "info and out must be non-null");
     isti_status st = validate(r, info, out_pitch);
     if (st != ISTI_OK) return st;
+    if (ext_base) {
+        const size_t need = isti_internal_scratch_bytes(r, info);
+        if (need > ext_bytes)
+            return fail(r, ISTI_ERR_UNSUPPORTED_SHAPE,
+                        "frame needs " + std::to_string(need) + " bytes of scratch, more than the " +
+                            std::to_string(ext_bytes) + " available");
+    }
     hipError_t e = hipSetDevice(r->device);
     if (e != hipSuccess) return dev_fail(r, e, "hipSetDevice");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    char *base = static_cast<char *>(ext_base);
 
-    if (info->mode == ISTI_MODE_TIMEWARP) return render_timewarp(r, info, out, out_pitch, s);
+    if (info->mode == ISTI_MODE_TIMEWARP) {
+        if (!base) return render_timewarp(r, info, out, out_pitch, s);
+        double *rows = reinterpret_cast<double *>(base);
+        float *mean = reinterpret_cast<float *>(base + align256(sizeof(double) * r->views[0][0].height));
+        return render_timewarp(r, info, out, out_pitch, s, rows, mean);
+    }
 
     FramePlan P;
-    st = plan_frame(r, info, out, out_pitch, P);
+    Scratch carved[2][MAX_VIEWS];
+    if (base) {
+        size_t off = 0;
+        for (int sl = 0; sl < 2; sl++)
+            for (int v = 0; v <= r->n_aux; v++) {
+                Scratch &c = carved[sl][v];
+                c.w = r->views[sl][v].width;
+                c.h = r->views[sl][v].height;
+                const size_t n = (size_t)c.w * c.h;
+                c.splat = reinterpret_cast<unsigned long long *>(base + off);
+                off += align256(n * sizeof(unsigned long long));
+                c.cam_ff = reinterpret_cast<float4 *>(base + off);
+                off += align256(n * sizeof(float4));
+                c.full_ff = reinterpret_cast<Flow3 *>(base + off);
+                off += align256(n * sizeof(Flow3));
+            }
+    }
+    st = plan_frame(r, info, out, out_pitch, P, base ? carved : nullptr);
     if (st != ISTI_OK) return st;
     const int shape = tile_shape_for(r, P);
     set_tiles(P.ra, shape);
@@ -880,6 +953,13 @@ isti_status isti_render(isti_renderer *r, const isti_render_info *info, uint8_t 
     if (e != hipSuccess) return dev_fail(r, e, "resolve launch");
     if ((e = mark(r, 3, s)) != hipSuccess) return dev_fail(r, e, "event");
     return ISTI_OK;
+}
+
+extern "C" {
+
+isti_status isti_render(isti_renderer *r, const isti_render_info *info, uint8_t *out, size_t out_pitch,
+                        void *stream) {
+    return isti_internal_render(r, info, out, out_pitch, stream, nullptr, 0);
 }
 
 isti_status isti_scratch(const isti_renderer *r, int32_t slot, int32_t view, isti_scratch_view *out) {

@@ -179,6 +179,26 @@ isti_status isti_keyframe_swap(isti_renderer *r);
 isti_status isti_render(isti_renderer *r, const isti_render_info *info,
                         uint8_t *out_rgba8, size_t out_pitch_bytes, void *stream);
 
+/* Size of the static device scratch of isti_render_noalloc, per GPU. */
+#define ISTI_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/* isti_render that allocates nothing. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, contract and results as isti_render, but every scratch plane of
+ * the frame is carved from one static device array of
+ * ISTI_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends) instead of
+ * the renderer's own buffers. A frame needs 36 bytes per source pixel of
+ * every keyframe view (rounded up to 256 bytes per plane; about 64 MiB holds
+ * a 1280x720 main view pair), timewarp 8 bytes per keyframe row; a larger
+ * frame returns ISTI_ERR_UNSUPPORTED_SHAPE before any device work. Every call
+ * recomputes the keyframe pair's flow (the array keeps only the last call's
+ * contents). All calls on one GPU share the array: calls whose work overlaps
+ * on that GPU (other streams, other threads) race and must be serialised by
+ * the caller. */
+isti_status isti_render_noalloc(isti_renderer *r, const isti_render_info *info,
+                                uint8_t *out_rgba8, size_t out_pitch_bytes, void *stream);
+
 /* Resolve tile shapes (render pixels per workgroup, 256 each): 0 = 16x16,
  * 1 = 32x8, 2 = 64x4, 3 = 8x32. Every shape gives identical results; they
  * differ in speed (halo staging, gather locality, wave shape). No counterpart
