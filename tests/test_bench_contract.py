@@ -2,8 +2,8 @@
 with a message, the algorithmic bytes behind roofline.achieved match the
 per-unit figures (clear 8 B per splat cell, splat 24 B per source pixel,
 resolve 12 B per source pixel, +4 with shadows, and 4 B per output pixel)
-for the 1080p + 2 aux workload, and the traffic file it reads names every
-stage."""
+for the 1080p + 2 aux workload, and the counter summary it reads covers
+every kernel variant it can run."""
 import json
 import subprocess
 import sys
@@ -34,9 +34,19 @@ def test_algorithmic_bytes_of_the_1080p_workload():
     assert bench.algorithmic_bytes(src, main, False)["resolve"] == 12 * src + 4 * main
 
 
-def test_traffic_file_names_every_stage():
-    t = json.loads((ROOT / "profiles" / "traffic_1080p.json").read_text())
-    for k in ("clear", "splat", "resolve"):
-        assert t[k] > 0
-    # measured bytes are at least the compulsory ones
-    assert t["clear"] >= 49_766_400 * 0.99 and t["resolve"] >= 107_827_200
+def test_counter_summary_covers_every_variant_the_bench_can_run():
+    """The committed rocprofv3 summary holds traffic and VALU counts for the
+    clear, both splat variants and every resolve tile shape of the 1080p
+    workload (2 aux cameras, window 1), under the kernel names bench.py asks
+    for; measured traffic is at least the compulsory bytes."""
+    sys.path.insert(0, str(ROOT))
+    import bench
+
+    c = json.loads((ROOT / "profiles" / "counters_1080p.json").read_text())["kernels"]
+    names = ["clear_splat_kernel", "reproject_splat_kernel", "reproject_splat_binned_kernel"]
+    names += [bench.resolve_kernel_name(1, False, 2, t) for t in ((16, 16), (32, 8), (64, 4), (8, 32))]
+    for n in names:
+        assert c[n]["traffic"] > 0 and c[n]["valu_insts"] > 0, n
+    assert c["clear_splat_kernel"]["traffic"] >= 49_766_400 * 0.99
+    for t in ((16, 16), (8, 32)):
+        assert c[bench.resolve_kernel_name(1, False, 2, t)]["traffic"] >= 107_827_200
