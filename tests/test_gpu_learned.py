@@ -167,24 +167,24 @@ def test_short_training_lowers_the_loss_and_checkpoint_round_trip(tmp_path):
 
 @pytest.mark.skipif(not Ln.WEIGHTS.exists(), reason="no trained weights in the package")
 def test_trained_network_against_plain_search_and_untrained():
-    """The shipped network at 1920x1080 (network grid 480x272) on a held-out
-    scene, against the plain search and against the same network untrained.
-    Training raises the share of searches that keep the learned start and
-    lowers the search steps below the untrained network's and the plain
-    search's; the plain search's judgement keeps the image within 0.1 dB of
-    the plain output's PSNR. The step saving is small by nature: the plain
-    window start already sits at the search's fixed point almost everywhere
-    on the main camera."""
+    """The shipped network at 1920x1080 (network grid 480x272) on three
+    held-out scenes, against the plain search and against the same network
+    untrained. What holds on every seed: training raises the share of
+    searches that keep their learned start well above the untrained
+    network's, and the plain search's judgement keeps the image within 0.1 dB
+    of the plain output's PSNR against ground truth. (Search steps saved are
+    not asserted: the plain window start already sits at the fixed point
+    almost everywhere, so the difference is a few hundredths of a step either
+    way.)"""
     params, meta = Ln.load_params(Ln.WEIGHTS)
-    trained = Ln.evaluate(params, 1920, 1080, 2, seeds=(900003,), pairs_per_seed=1)
     untrained_params = Ln.UNet(8, 8, batch=1, seed=7).get_params()
-    untrained = Ln.evaluate(untrained_params, 1920, 1080, 2, seeds=(900003,), pairs_per_seed=1)
-    print("trained", trained)
-    print("untrained", untrained)
-    assert trained["kept_rate"] > 0.8, trained
-    assert trained["kept_rate"] > untrained["kept_rate"] + 0.05, (trained, untrained)
-    assert trained["steps_per_search_learned"] < untrained["steps_per_search_learned"], (trained, untrained)
-    assert trained["steps_per_search_learned"] < trained["steps_per_search_plain"], trained
-    assert abs(trained["delta_psnr_db"]) <= 0.1, trained
-    assert trained["psnr_learned_vs_plain"] >= 40.0, trained
-    assert abs(untrained["delta_psnr_db"]) <= 0.1, untrained  # the fallback guards quality either way
+    for seed in (900001, 900002, 900003):
+        trained = Ln.evaluate(params, 1920, 1080, 2, seeds=(seed,), pairs_per_seed=1)
+        untrained = Ln.evaluate(untrained_params, 1920, 1080, 2, seeds=(seed,), pairs_per_seed=1)
+        print("seed", seed, "trained", trained)
+        print("seed", seed, "untrained", untrained)
+        assert trained["kept_rate"] > 0.8, trained
+        assert trained["kept_rate"] > untrained["kept_rate"] + 0.05, (trained, untrained)
+        assert abs(trained["delta_psnr_db"]) <= 0.1, trained
+        assert trained["psnr_learned_vs_plain"] >= 40.0, trained
+        assert abs(untrained["delta_psnr_db"]) <= 0.1, untrained  # the fallback guards quality either way
