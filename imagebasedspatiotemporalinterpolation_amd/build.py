@@ -65,7 +65,7 @@ def build_native(verbose: bool = False) -> None:
     obj_dir.mkdir(exist_ok=True)
     cmds, objs = [], []
     for src in ("resolve_kernels.hip", "isti_kernels.hip", "unet.hip", "quality.hip", "isti_noalloc.hip",
-                "isti_api.cpp"):
+                "codec.hip", "isti_api.cpp"):
         obj = obj_dir / (src.rsplit(".", 1)[0] + ".o")
         extra = ["-x", "hip"] if src.endswith(".hip") else []
         cmds.append([HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", *extra, "-c", CSRC / src, "-o", obj])

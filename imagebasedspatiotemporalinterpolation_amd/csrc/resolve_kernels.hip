@@ -6,11 +6,12 @@
 
 namespace isti {
 
-// Occupancy floor per variant (waves per SIMD). The pass is bound by the
-// latency of its dependent gathers, so waves in flight pay: the 2-aux
-// window-1 variant (1080p + 2 aux workload) fits 64 VGPRs without spilling
-// and runs at 8 waves, the 3-aux one at 6 (79 VGPRs). A floor that forces
-// spills costs more than the waves gain. 1 = no floor.
+// Occupancy floor per variant (waves per SIMD). The pass spends most of its
+// time issuing VALU instructions (about three quarters of a launch on the
+// 1080p + 2 aux workload) and the rest waiting on dependent gathers, so
+// waves in flight pay: the 2-aux window-1 variant fits 64 VGPRs without
+// spilling and runs at 8 waves, the 3-aux one at 6 (79 VGPRs). A floor that
+// forces spills costs more than the waves gain. 1 = no floor.
 constexpr int resolve_min_waves(int w, int naux) { return w != 1 ? 1 : (naux == 2 ? 8 : (naux == 3 ? 6 : 1)); }
 
 template <int W, bool MAV, int NAUX, int TW, int TH>

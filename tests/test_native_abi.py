@@ -19,6 +19,7 @@ def _declared(header: Path):
     ("include/isti.h", "imagebasedspatiotemporalinterpolation_amd/libisti.so"),
     ("include/isti_learned.h", "imagebasedspatiotemporalinterpolation_amd/libisti.so"),
     ("include/isti_quality.h", "imagebasedspatiotemporalinterpolation_amd/libisti.so"),
+    ("include/isti_codec.h", "imagebasedspatiotemporalinterpolation_amd/libisti.so"),
     ("include/isti_io.h", "imagebasedspatiotemporalinterpolation_amd/libisti_io.so"),
     ("include/isti_scene.h", "imagebasedspatiotemporalinterpolation_amd/libisti_scene.so"),
 ])
