@@ -270,17 +270,38 @@ def learned_bench(n_frames: int, seed: int, iters: int) -> dict:
 
     plain = timed(lambda ri: r.render(out, ri))
     learned = timed(lambda ri: li.render(r, out, ri))
-    # the network alone (features already built for the last frame)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(20):
+    # per stage of the learned path: splat + features, network, learned resolve
+    n_st = 3 * len(infos)
+    marks = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(n_st)]
+    for i in range(n_st):
+        ri = infos[i % len(infos)]
+        marks[i][0].record(stream)
+        Ln.features(r, ri, li.x)
+        marks[i][1].record(stream)
         li.net.forward(li.x, li.out)
-    e1.record(stream)
+        marks[i][2].record(stream)
+        Ln.render_learned(r, out, ri, li.out)
+        marks[i][3].record(stream)
     torch.cuda.synchronize()
-    net_ms = e0.elapsed_time(e1) / 20
+    stage = np.array([[m[k].elapsed_time(m[k + 1]) for k in range(3)] for m in marks]).mean(axis=0)
+    net_ms = float(stage[1])
     li.close()
     r.close()
-    ev = Ln.evaluate(params, W, H, shift, seeds=(seed + 900000,), pairs_per_seed=1, max_iter=iters)
+    per_seed = []
+    for sd in (seed + 900000, seed + 900001, seed + 900002):  # scenes outside the training set
+        e = Ln.evaluate(params, W, H, shift, seeds=(sd,), pairs_per_seed=1, max_iter=iters)
+        per_seed.append({"seed": sd,
+                         "steps_per_search_plain": round(e["steps_per_search_plain"], 4),
+                         "steps_per_search_learned": round(e["steps_per_search_learned"], 4),
+                         "learned_start_rate": round(e["learned_start_rate"], 4),
+                         "fallback_rate": round(e["fallback_rate"], 4),
+                         "psnr_plain_vs_gt_db": round(e["psnr_plain_vs_gt"], 3),
+                         "psnr_learned_vs_gt_db": round(e["psnr_learned_vs_gt"], 3),
+                         "delta_psnr_db": round(e["delta_psnr_db"], 4)})
+    ev = {k: float(np.mean([p[k] for p in per_seed])) for k in
+          ("steps_per_search_plain", "steps_per_search_learned", "learned_start_rate", "fallback_rate",
+           "delta_psnr_db")}
+    ev["iterations_saved_per_search"] = ev["steps_per_search_plain"] - ev["steps_per_search_learned"]
     return {
         "workload": "C1 scene generator at 1920x1080 (checker plane + 16 spheres), fpa 4, mode 0, main camera; "
                     "network trained on 4000 seeded 960x540 pairs, run at 480x272",
@@ -288,6 +309,8 @@ def learned_bench(n_frames: int, seed: int, iters: int) -> dict:
         "learned": learned,
         "speedup": round(learned["mpix_per_s"] / plain["mpix_per_s"], 3),
         "network_ms": round(net_ms, 4),
+        "learned_stages_ms": {"splat_and_features": round(float(stage[0]), 4), "network": round(float(stage[1]), 4),
+                              "learned_resolve": round(float(stage[2]), 4)},
         "network_gflop": round(83.5e3 * 2 * li.net_w * li.net_h / 1e9, 1),
         "iterations_saved_per_search": round(ev["iterations_saved_per_search"], 4),
         "steps_per_search_plain": round(ev["steps_per_search_plain"], 4),
@@ -295,7 +318,7 @@ def learned_bench(n_frames: int, seed: int, iters: int) -> dict:
         "fallback_rate": round(ev["fallback_rate"], 4),
         "learned_start_rate": round(ev["learned_start_rate"], 4),
         "delta_psnr_db": round(ev["delta_psnr_db"], 4),
-        "psnr_learned_vs_plain_db": round(ev["psnr_learned_vs_plain"], 2),
+        "held_out_seeds": per_seed,
         "train_loss_last100": meta.get("loss_last100"),
     }
 
@@ -513,10 +536,17 @@ def main():
         from imagebasedspatiotemporalinterpolation_amd import serving
 
         print("[bench] serving mix", file=sys.stderr, flush=True)
+        store = serving.KeyframeStore(args.seed)
+        mix = ("48 sessions from {640x360, 1280x720, 1920x1080, stereo 2x2064x2208} "
+               "(weights .3/.3/.25/.15), Poisson arrivals 90 Hz per session, mode 0")
         res["serving"] = serving.run_mix(duration_s=args.serving_seconds, seed=args.seed, n_streams=3,
-                                         max_batch=16, graph=True)
-        res["serving"]["workload"] = ("48 sessions from {640x360, 1280x720, 1920x1080, stereo 2x2064x2208} "
-                                      "(weights .3/.3/.25/.15), Poisson arrivals 90 Hz per session, mode 0")
+                                         max_batch=16, graph=True, store=store)
+        res["serving"]["workload"] = mix
+        if args.learned_frames > 0:
+            print("[bench] serving mix, learned starts", file=sys.stderr, flush=True)
+            res["serving_learned"] = serving.run_mix(duration_s=args.serving_seconds, seed=args.seed, n_streams=3,
+                                                     max_batch=16, store=store, search="learned")
+            res["serving_learned"]["workload"] = mix
     if not args.no_cpu_baseline:
         print("[bench] timing the CPU oracle sample", file=sys.stderr, flush=True)
         res["cpu_baseline"] = cpu_baseline(fwd, bwd, infos, args.cpu_seconds)

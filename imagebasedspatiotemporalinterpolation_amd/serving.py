@@ -22,6 +22,10 @@ requests with continuous batching:
     streams overlap on the device;
   - a session's requests run in arrival order (one in flight at a time,
     since they share the session's render state);
+  - with search="learned" every request takes the learned-start path
+    (splat + features, the network, the learned resolve; learned.py) on its
+    stream instead of the batched plain launch, so the two methods can be
+    served side by side on the same mix;
   - latency is measured from arrival to the host observing completion.
 
 Results do not depend on the mix: a request renders exactly what it would
@@ -218,6 +222,16 @@ class Stats:
         }
 
 
+def learned_shift(width: int, height: int) -> int:
+    """Reduction (as a power of two, at least 2x) from a session's size to
+    the learned path's network grid: the smallest that keeps the grid within
+    512 cells a side (1920x1080 runs the network at 480 x 272)."""
+    k = 1
+    while -(-width // (1 << k)) > 512 or -(-height // (1 << k)) > 512:
+        k += 1
+    return k
+
+
 class Server:
     """Admission by HBM footprint + continuous batching over several streams.
 
@@ -226,8 +240,19 @@ class Server:
     class, when its first session is admitted)."""
 
     def __init__(self, hbm_budget: int, max_batch: int = 32, n_streams: int = 2, graph: bool = True,
-                 max_iter: int = -1, store: Optional[KeyframeStore] = None):
+                 max_iter: int = -1, store: Optional[KeyframeStore] = None, search: str = "plain",
+                 learned_params=None):
         import torch
+
+        if search not in ("plain", "learned"):
+            raise ValueError("search is 'plain' or 'learned'")
+        self.search = search
+        self._params = learned_params
+        if search == "learned" and learned_params is None:
+            from . import learned as Ln
+
+            self._params, _ = Ln.load_params(Ln.WEIGHTS)
+        self._learned: Dict[tuple, object] = {}  # (class name, stream) -> LearnedInit
 
         self.budget = hbm_budget
         self.used = 0
@@ -309,7 +334,7 @@ class Server:
         self.inflight[i] = None
 
     def _launch(self, i: int, now: float):
-        chosen, work, seen = [], [], set()
+        chosen, work, owners, seen = [], [], [], set()
         keep = collections.deque()
         while self.pending and len(chosen) < self.max_batch:
             r = self.pending.popleft()
@@ -328,12 +353,17 @@ class Server:
             outs = r.outs if r.outs is not None else s.outs
             for e, rend in enumerate(s.renderers):
                 work.append((rend, s.info(r.pair, r.b, e, self.max_iter), outs[e]))
+                owners.append(s.cls)
             r.launched = now
             self.busy.add(r.sid)
         import torch
 
         with torch.cuda.stream(self.streams[i]):
-            self.batches[i].render(work, stream=self.streams[i])
+            if self.search == "learned":
+                for cls, (rend, info, out) in zip(owners, work):
+                    self.learned_init(cls, i).render(rend, out, info, stream=self.streams[i])
+            else:
+                self.batches[i].render(work, stream=self.streams[i])
             ev = torch.cuda.Event()
             ev.record(self.streams[i])
         self.inflight[i] = (ev, chosen)
@@ -360,6 +390,19 @@ class Server:
             self.close_session(sid)
         for b in self.batches:
             b.close()
+        for li in self._learned.values():
+            li.close()
+        self._learned = {}
+
+    def learned_init(self, cls: SessionClass, i: int):
+        """The learned-start network state of stream i for sessions of cls."""
+        from . import learned as Ln
+
+        key = (cls.name, i)
+        if key not in self._learned:
+            self._learned[key] = Ln.LearnedInit(cls.width, cls.height, learned_shift(cls.width, cls.height),
+                                                self._params)
+        return self._learned[key]
 
 
 def poisson_arrivals(rate_hz: float, duration_s: float, rng) -> np.ndarray:
@@ -377,7 +420,7 @@ def frame_schedule(n: int, fpa: int, phase: int, n_pairs: int) -> tuple:
 
 def run_mix(n_sessions: int = 48, rate_hz: float = 90.0, duration_s: float = 2.0, seed: int = 1,
             hbm_budget: int = 64 << 30, max_batch: int = 32, n_streams: int = 2, graph: bool = True,
-            max_iter: int = -1, store: Optional[KeyframeStore] = None) -> dict:
+            max_iter: int = -1, store: Optional[KeyframeStore] = None, search: str = "plain") -> dict:
     """Serving mix: sessions drawn from SERVING_MIX, Poisson arrivals per
     session; every session walks through the store's keyframe pairs."""
     rng = np.random.default_rng(seed)
@@ -387,7 +430,8 @@ def run_mix(n_sessions: int = 48, rate_hz: float = 90.0, duration_s: float = 2.0
     store = store or KeyframeStore(seed)
     for k in sorted(set(picks)):
         store.preload(classes[k])
-    srv = Server(hbm_budget, max_batch=max_batch, n_streams=n_streams, graph=graph, max_iter=max_iter, store=store)
+    srv = Server(hbm_budget, max_batch=max_batch, n_streams=n_streams, graph=graph, max_iter=max_iter, store=store,
+                 search=search)
     phases = rng.integers(0, store.n_pairs, n_sessions)
     sids = [srv.open_session(classes[k], int(p)) for k, p in zip(picks, phases)]
     events = []
@@ -422,6 +466,6 @@ def run_mix(n_sessions: int = 48, rate_hz: float = 90.0, duration_s: float = 2.0
                                             * rate_hz / 1e6, 1),
                 "pair_advances": int(sum(s.advances for s in active)),
                 "hbm_used_bytes": int(srv.used),
-                "streams": n_streams, "graph": graph})
+                "streams": n_streams, "graph": graph and search == "plain", "search": search})
     srv.close()
     return out
