@@ -232,20 +232,21 @@ def learned_bench(n_frames: int, seed: int, iters: int) -> dict:
     r = R.Renderer(0)
     fd = R.FrameData(R.KeyframeView.from_numpy(fwd), R.KeyframeView.from_numpy(bwd))
     r.set_frame(fd)
-    infos = []
+    infos, infos_1 = [], []
     for b in range(1, fpa):
         vp, ivp, _, _ = scene.camera(cfg, b)
-        infos.append(R.RenderInfo((W, H), b / fpa, vp, ivp, R.AdditionalRenderInfo(fpa, 0, True, R.RenderMode.OURS),
-                                  iters))
+        add = R.AdditionalRenderInfo(fpa, 0, True, R.RenderMode.OURS)
+        infos.append(R.RenderInfo((W, H), b / fpa, vp, ivp, add, iters))
+        infos_1.append(R.RenderInfo((W, H), b / fpa, vp, ivp, add, 1))
     li = Ln.LearnedInit(W, H, shift, params)
     out = torch.zeros((H, W, 4), dtype=torch.uint8, device="cuda")
     stream = torch.cuda.current_stream()
 
-    def timed(fn):
+    def timed(fn, frames=infos):
         def step(i):
-            if i % len(infos) == 0:  # a new keyframe pair: its flows are computed once more
+            if i % len(frames) == 0:  # a new keyframe pair: its flows are computed once more
                 r.set_frame(fd)
-            fn(infos[i % len(infos)])
+            fn(frames[i % len(frames)])
 
         for i in range(10):
             step(i)
@@ -269,6 +270,10 @@ def learned_bench(n_frames: int, seed: int, iters: int) -> dict:
                 "p99_frame_ms": round(float(np.percentile(ms, 99)), 4)}
 
     plain = timed(lambda ri: r.render(out, ri))
+    # the plain frame with every search stopped after its first step: the
+    # floor of any start predictor that keeps the splat and checks its start
+    # with one step, even with a network that costs nothing
+    one_step = timed(lambda ri: r.render(out, ri), infos_1)
     learned = timed(lambda ri: li.render(r, out, ri))
     # per stage of the learned path: splat + features, network, learned resolve
     n_st = 3 * len(infos)
@@ -308,6 +313,8 @@ def learned_bench(n_frames: int, seed: int, iters: int) -> dict:
         "plain": plain,
         "learned": learned,
         "speedup": round(learned["mpix_per_s"] / plain["mpix_per_s"], 3),
+        "plain_one_step": one_step,
+        "speedup_bound_free_network": round(one_step["mpix_per_s"] / plain["mpix_per_s"], 3),
         "network_ms": round(net_ms, 4),
         "learned_stages_ms": {"splat_and_features": round(float(stage[0]), 4), "network": round(float(stage[1]), 4),
                               "learned_resolve": round(float(stage[2]), 4)},
