@@ -6,7 +6,7 @@
  *   - the lossless ShadowInfo path (compress_raw_images_lossless :223-279,
  *     decompress_raw_images_lossless :281-325): per frame the third colour
  *     channel of the ShadowInfo image, optionally XOR-ed with the anchor
- *     frame taken every server_fps frames (:251-255, :311-315), optionally
+ *     frame taken every server_fps frames (:249-252, :316-319), optionally
  *     run-length coded (rle_encode :176-204, rle_decode :206-218), framed as
  *     [u16 name length][name][u32 width][u32 height][u32 length][bytes]
  *     (little endian, :262-268) inside one zstd stream. The byte kernels are

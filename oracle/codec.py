@@ -41,7 +41,7 @@ def rle_decode(data: bytes) -> bytes:
 
 
 def xor_delta(frames, server_fps: int):
-    """compression.py:251-255: frame k is kept when k % server_fps == 0 (the
+    """compression.py:249-252: frame k is kept when k % server_fps == 0 (the
     anchor) and XOR-ed with the latest anchor otherwise."""
     out, anchor = [], None
     for k, f in enumerate(frames):

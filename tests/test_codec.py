@@ -1,6 +1,6 @@
 """The keyframe wire codec's CPU side: the oracle's run-length coder, XOR
 delta and 16-bit normalisation pinned by hand-computed vectors, closed forms
-and round trips (compression.py:84-98, 176-218, 251-255), and the record
+and round trips (compression.py:84-98, 176-218, 249-252), and the record
 framing and zstd container of codec.py (host code, libzstd)."""
 import struct
 from fractions import Fraction
