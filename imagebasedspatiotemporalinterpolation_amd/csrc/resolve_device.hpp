@@ -152,7 +152,37 @@ __device__ __forceinline__ float reproj_error(uint2 q, const ResolveView &v, flo
 struct Texels {
     uchar4 img, info, sa, sb;
 };
-__device__ __forceinline__ Texels load_texels(const ResolveView &v, uint32_t px, uint32_t py, bool shadows) {
+
+// The fields of a view the colour stage reads, taken per lane from one of two
+// views (the keyframe a lane's candidate came from, or the other one). Lanes
+// of a wave that chose different keyframes then share one pass through the
+// colour stage instead of running it once per keyframe.
+struct LaneView {
+    const uchar4 *img, *sh_info, *sh_a, *sh_b;
+    const float *z;
+    const Flow3 *full_ff;
+    uint32_t w, h;
+    double inv_w, inv_h;
+    int32_t forward;
+};
+__device__ __forceinline__ LaneView lane_view(bool second, const ResolveView &a, const ResolveView &b) {
+    LaneView v;
+    v.img = second ? b.img : a.img;
+    v.sh_info = second ? b.sh_info : a.sh_info;
+    v.sh_a = second ? b.sh_a : a.sh_a;
+    v.sh_b = second ? b.sh_b : a.sh_b;
+    v.z = second ? b.z : a.z;
+    v.full_ff = second ? b.full_ff : a.full_ff;
+    v.w = second ? b.w : a.w;
+    v.h = second ? b.h : a.h;
+    v.inv_w = second ? b.inv_w : a.inv_w;
+    v.inv_h = second ? b.inv_h : a.inv_h;
+    v.forward = second ? b.forward : a.forward;
+    return v;
+}
+
+template <typename V>
+__device__ __forceinline__ Texels load_texels(const V &v, uint32_t px, uint32_t py, bool shadows) {
     Texels t;
     t.img = fetch(v.img, v.w, v.h, px, py);
     t.info = t.sa = t.sb = make_uchar4(0, 0, 0, 0);
@@ -166,12 +196,14 @@ __device__ __forceinline__ Texels load_texels(const ResolveView &v, uint32_t px,
 
 // colorFromShadow(idcs, t, ...) reads shadows only for these frames and views
 // (renderer.cu:272-273): otherwise it returns the plain colour.
-__device__ __forceinline__ bool uses_shadows(uint2 idcs, int sfpa, bool no_shadows, const ResolveView &v) {
+template <typename V>
+__device__ __forceinline__ bool uses_shadows(uint2 idcs, int sfpa, bool no_shadows, const V &v) {
     return !(no_shadows || idcs.x >= (uint32_t)sfpa || idcs.y == 0 || v.sh_a == nullptr);
 }
 
 // colorFromShadow(idx, ...), renderer.cu:217-268, on loaded texels
-__device__ __forceinline__ uchar4 color_from_shadow_1(uint32_t vidx, int sfpa, const ResolveView &v, const Texels &t) {
+template <typename V>
+__device__ __forceinline__ uchar4 color_from_shadow_1(uint32_t vidx, int sfpa, const V &v, const Texels &t) {
     const uchar4 main_color = t.img;
     if (vidx < (uint32_t)sfpa && vidx > 0) {
         const uchar4 si = t.info;
@@ -206,8 +238,9 @@ __device__ __forceinline__ uchar4 color_from_shadow_1(uint32_t vidx, int sfpa, c
 
 // colorFromShadow(idcs, t, ...), renderer.cu:270-295 (frameIdx2Idx :227-230),
 // on texels loaded with load_texels(..., uses_shadows(...))
-__device__ __forceinline__ uchar4 color_from_shadow(uint2 idcs, float st, int sfpa, bool no_shadows,
-                                                    const ResolveView &v, const Texels &t) {
+template <typename V>
+__device__ __forceinline__ uchar4 color_from_shadow(uint2 idcs, float st, int sfpa, bool no_shadows, const V &v,
+                                                    const Texels &t) {
     if (!uses_shadows(idcs, sfpa, no_shadows, v)) return t.img;
     const uint32_t vx = (uint32_t)(v.forward ? (int)idcs.x : sfpa - (int)idcs.x);
     const uint32_t vy = (uint32_t)(v.forward ? (int)idcs.y : sfpa - (int)idcs.y);
@@ -230,8 +263,9 @@ struct FrameConst {
 // secondChanceAndBlend_kernel<EPS_2>, renderer.cu:297-327. The other
 // keyframe's depth, flow and texels at the projected pixel are loaded in one
 // round trip, before the similarity test decides whether they are used.
-__device__ __forceinline__ uchar4 second_chance(float weight, const FrameConst &fc, SplatInfo si,
-                                                const ResolveView &from, const ResolveView &to, float eps2) {
+template <typename V>
+__device__ __forceinline__ uchar4 second_chance(float weight, const FrameConst &fc, SplatInfo si, const V &from,
+                                                const V &to, float eps2) {
     const bool sh_from = uses_shadows(fc.sidx, fc.sfpa, fc.no_shadows, from);
     const bool sh_to = uses_shadows(fc.sidx, fc.sfpa, fc.no_shadows, to);
     const Flow3 m_ft = fetch(from.full_ff, from.w, from.h, si.x, si.y);
@@ -291,11 +325,22 @@ __device__ __forceinline__ Cand decide(float fe, float be, float fd, float bd, b
     return k;
 }
 
-// Colour of a candidate of camera (fwd, bwd): renderer.cu:378-396.
+// Colour of a candidate of camera (fwd, bwd): renderer.cu:378-396. With
+// LANE_VIEWS the keyframe is picked per lane (one pass through the colour
+// stage for a wave whose lanes chose different keyframes, at the price of
+// per-lane addresses); without, each keyframe's pass runs with scalar
+// addresses for the lanes that chose it.
+template <bool LANE_VIEWS>
 __device__ __forceinline__ uchar4 cand_colour(const Cand &k, const FrameConst &fc, SplatInfo fi, SplatInfo bi,
                                               const ResolveView &fwd, const ResolveView &bwd, float eps2) {
-    if (k.dir == 0) return second_chance(1.0f - fc.alpha, fc, fi, fwd, bwd, eps2);
-    if (k.dir == 1) return second_chance(fc.alpha, fc, bi, bwd, fwd, eps2);
+    if (!LANE_VIEWS) {
+        if (k.dir == 0) return second_chance(1.0f - fc.alpha, fc, fi, fwd, bwd, eps2);
+        if (k.dir == 1) return second_chance(fc.alpha, fc, bi, bwd, fwd, eps2);
+    } else if (k.dir != 2) {
+        const bool b = k.dir == 1;
+        return second_chance(b ? fc.alpha : 1.0f - fc.alpha, fc, b ? bi : fi, lane_view(b, fwd, bwd),
+                             lane_view(b, bwd, fwd), eps2);
+    }
     const uchar4 cf = second_chance(1.0f - fc.alpha, fc, fi, fwd, bwd, eps2);
     const uchar4 cb = second_chance(fc.alpha, fc, bi, bwd, fwd, eps2);
     return lerp_to_u8(cf, cb, 0.5f);
@@ -626,11 +671,12 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
     // per lane); a wave runs the branches its lanes chose.
     uchar4 col = make_uchar4(0, 0, 0, 0);
     if constexpr (NC == 1) {
-        col = cand_colour(wk, fc, wf, wb, a.fwd[0], a.bwd[0], EPS_2_FULL_RES);
+        col = cand_colour<false>(wk, fc, wf, wb, a.fwd[0], a.bwd[0], EPS_2_FULL_RES);
     } else {
 #pragma unroll
         for (int c = 0; c < NC; c++)
-            if (wc == c) col = cand_colour(wk, fc, wf, wb, a.fwd[c], a.bwd[c], c == 0 ? EPS_2_FULL_RES : EPS_2_LOW_RES);
+            if (wc == c)
+                col = cand_colour<true>(wk, fc, wf, wb, a.fwd[c], a.bwd[c], c == 0 ? EPS_2_FULL_RES : EPS_2_LOW_RES);
     }
     a.out[y * a.out_pitch_px + x] = col;
 }
