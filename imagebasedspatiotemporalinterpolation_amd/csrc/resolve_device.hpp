@@ -110,30 +110,26 @@ __device__ __forceinline__ void chain_start(Chain &c, bool run, uint32_t x, uint
 // Step i (0-based) of n_iter: q_{i+1} = step(q_i), with the exits described above.
 __device__ __forceinline__ void chain_step(Chain &c, int i, int n_iter, float2 pcs_full, const ResolveView &v) {
     if (!c.active) return;
+    // The exits are evaluated side by side and applied with selects, so the
+    // common paths (advance, or stop) take no nested branches; only the rare
+    // cycle jump and the gather of an advancing chain are branched.
     const int2 t = cs2pixel(pcs_full.x - c.m.x, pcs_full.y - c.m.y, v.w, v.h);
-    if (!in_range_u(t, v.w, v.h)) {
-        c.active = false;
-        return;
-    }
-    const uint32_t nt = pack_px((uint32_t)t.x, (uint32_t)t.y);
-    if (nt == c.h0) {
-        c.active = false;
-        return;
-    }
-    // three compares whose masks are OR-ed, without a short-circuit branch
-    if ((int)(nt == c.h1) | (int)(nt == c.h2) | (int)(nt == c.h3)) {
+    const bool inr = in_range_u(t, v.w, v.h);
+    const uint32_t nt = pack_px((uint32_t)t.x, (uint32_t)t.y);  // meaningful where inr
+    const bool moved = inr && nt != c.h0;
+    const bool cyc = moved && ((int)(nt == c.h1) | (int)(nt == c.h2) | (int)(nt == c.h3));
+    if (cyc) {
         const int p = nt == c.h1 ? 2 : (nt == c.h2 ? 3 : 4);
         const int idx = p - 1 - (n_iter - (i + 1)) % p;  // c_r = h_{p-1-r}
-        const uint32_t q = idx == 0 ? c.h0 : (idx == 1 ? c.h1 : (idx == 2 ? c.h2 : c.h3));
-        c.h0 = q;  // the chain stops: m is not needed again
-        c.active = false;
-        return;
+        c.h0 = idx == 0 ? c.h0 : (idx == 1 ? c.h1 : (idx == 2 ? c.h2 : c.h3));  // the chain stops
     }
-    c.h3 = c.h2;
-    c.h2 = c.h1;
-    c.h1 = c.h0;
-    c.h0 = nt;
-    c.m = at_xy(v.cam_ff, v.w, (uint32_t)t.x, (uint32_t)t.y);
+    const bool adv = moved && !cyc;
+    c.active = adv;
+    c.h3 = adv ? c.h2 : c.h3;
+    c.h2 = adv ? c.h1 : c.h2;
+    c.h1 = adv ? c.h0 : c.h1;
+    c.h0 = adv ? nt : c.h0;
+    if (adv) c.m = at_xy(v.cam_ff, v.w, (uint32_t)t.x, (uint32_t)t.y);
 }
 
 // computeReprojError, renderer.cu:204-215
