@@ -244,9 +244,10 @@ def learned_bench(n_frames: int, seed: int, iters: int) -> dict:
 
     def timed(fn, frames=infos):
         def step(i):
-            if i % len(frames) == 0:  # a new keyframe pair: its flows are computed once more
+            first = i % len(frames) == 0
+            if first:  # a new keyframe pair: its flows are computed once more
                 r.set_frame(fd)
-            fn(frames[i % len(frames)])
+            fn(frames[i % len(frames)], first)
 
         for i in range(10):
             step(i)
@@ -269,12 +270,14 @@ def learned_bench(n_frames: int, seed: int, iters: int) -> dict:
                 "p50_frame_ms": round(float(np.percentile(ms, 50)), 4),
                 "p99_frame_ms": round(float(np.percentile(ms, 99)), 4)}
 
-    plain = timed(lambda ri: r.render(out, ri))
+    plain = timed(lambda ri, first: r.render(out, ri))
     # the plain frame with every search stopped after its first step: the
     # floor of any start predictor that keeps the splat and checks its start
     # with one step, even with a network that costs nothing
-    one_step = timed(lambda ri: r.render(out, ri), infos_1)
-    learned = timed(lambda ri: li.render(r, out, ri))
+    one_step = timed(lambda ri, first: r.render(out, ri), infos_1)
+    learned = timed(lambda ri, first: li.render(r, out, ri))
+    # the network once per keyframe pair (its first frame), reused by the rest
+    learned_pp = timed(lambda ri, first: li.render_pair_frame(r, out, ri, first))
     # per stage of the learned path: splat + features, network, learned resolve
     n_st = 3 * len(infos)
     marks = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(n_st)]
@@ -292,33 +295,40 @@ def learned_bench(n_frames: int, seed: int, iters: int) -> dict:
     net_ms = float(stage[1])
     li.close()
     r.close()
-    per_seed = []
-    for sd in (seed + 900000, seed + 900001, seed + 900002):  # scenes outside the training set
-        e = Ln.evaluate(params, W, H, shift, seeds=(sd,), pairs_per_seed=1, max_iter=iters)
-        per_seed.append({"seed": sd,
-                         "steps_per_search_plain": round(e["steps_per_search_plain"], 4),
-                         "steps_per_search_learned": round(e["steps_per_search_learned"], 4),
-                         "learned_start_rate": round(e["learned_start_rate"], 4),
-                         "fallback_rate": round(e["fallback_rate"], 4),
-                         "psnr_plain_vs_gt_db": round(e["psnr_plain_vs_gt"], 3),
-                         "psnr_learned_vs_gt_db": round(e["psnr_learned_vs_gt"], 3),
-                         "delta_psnr_db": round(e["delta_psnr_db"], 4)})
-    ev = {k: float(np.mean([p[k] for p in per_seed])) for k in
-          ("steps_per_search_plain", "steps_per_search_learned", "learned_start_rate", "fallback_rate",
-           "delta_psnr_db")}
-    ev["iterations_saved_per_search"] = ev["steps_per_search_plain"] - ev["steps_per_search_learned"]
+    def held_out(per_pair):
+        per_seed = []
+        for sd in (seed + 900000, seed + 900001, seed + 900002):  # scenes outside the training set
+            e = Ln.evaluate(params, W, H, shift, seeds=(sd,), pairs_per_seed=1, max_iter=iters, per_pair=per_pair)
+            per_seed.append({"seed": sd,
+                             "steps_per_search_plain": round(e["steps_per_search_plain"], 4),
+                             "steps_per_search_learned": round(e["steps_per_search_learned"], 4),
+                             "learned_start_rate": round(e["learned_start_rate"], 4),
+                             "fallback_rate": round(e["fallback_rate"], 4),
+                             "psnr_plain_vs_gt_db": round(e["psnr_plain_vs_gt"], 3),
+                             "psnr_learned_vs_gt_db": round(e["psnr_learned_vs_gt"], 3),
+                             "delta_psnr_db": round(e["delta_psnr_db"], 4)})
+        ev = {k: float(np.mean([p[k] for p in per_seed])) for k in
+              ("steps_per_search_plain", "steps_per_search_learned", "learned_start_rate", "fallback_rate",
+               "delta_psnr_db")}
+        ev["iterations_saved_per_search"] = ev["steps_per_search_plain"] - ev["steps_per_search_learned"]
+        return per_seed, ev
+
+    per_seed, ev = held_out(False)
+    per_seed_pp, ev_pp = held_out(True)
     return {
         "workload": "C1 scene generator at 1920x1080 (checker plane + 16 spheres), fpa 4, mode 0, main camera; "
                     "network trained on 4000 seeded 960x540 pairs, run at 480x272",
         "plain": plain,
         "learned": learned,
         "speedup": round(learned["mpix_per_s"] / plain["mpix_per_s"], 3),
+        "learned_network_per_pair": learned_pp,
+        "speedup_network_per_pair": round(learned_pp["mpix_per_s"] / plain["mpix_per_s"], 3),
         "plain_one_step": one_step,
         "speedup_bound_free_network": round(one_step["mpix_per_s"] / plain["mpix_per_s"], 3),
         "network_ms": round(net_ms, 4),
         "learned_stages_ms": {"splat_and_features": round(float(stage[0]), 4), "network": round(float(stage[1]), 4),
                               "learned_resolve": round(float(stage[2]), 4)},
-        "network_gflop": round(83.5e3 * 2 * li.net_w * li.net_h / 1e9, 1),
+        "network_gflop": round(Ln.flop_per_cell() * 2 * li.net_w * li.net_h / 1e9, 1),
         "iterations_saved_per_search": round(ev["iterations_saved_per_search"], 4),
         "steps_per_search_plain": round(ev["steps_per_search_plain"], 4),
         "steps_per_search_learned": round(ev["steps_per_search_learned"], 4),
@@ -326,6 +336,12 @@ def learned_bench(n_frames: int, seed: int, iters: int) -> dict:
         "learned_start_rate": round(ev["learned_start_rate"], 4),
         "delta_psnr_db": round(ev["delta_psnr_db"], 4),
         "held_out_seeds": per_seed,
+        "network_per_pair": {"iterations_saved_per_search": round(ev_pp["iterations_saved_per_search"], 4),
+                             "steps_per_search_learned": round(ev_pp["steps_per_search_learned"], 4),
+                             "fallback_rate": round(ev_pp["fallback_rate"], 4),
+                             "learned_start_rate": round(ev_pp["learned_start_rate"], 4),
+                             "delta_psnr_db": round(ev_pp["delta_psnr_db"], 4),
+                             "held_out_seeds": per_seed_pp},
         "train_loss_last100": meta.get("loss_last100"),
     }
 

@@ -704,7 +704,7 @@ isti_status isti_set_search_capture(isti_renderer *r, uint32_t *q_out) {
 isti_status isti_learned_features(isti_renderer *r, const isti_render_info *info, uint16_t *x, uint32_t net_w,
                                   uint32_t net_h, void *stream) {
     if (!r) return ISTI_ERR_INVALID_ARG;
-    if (!info || !x || net_w == 0 || net_h == 0) return fail(r, ISTI_ERR_INVALID_ARG, "bad features arguments");
+    if (!info || net_w == 0 || net_h == 0) return fail(r, ISTI_ERR_INVALID_ARG, "bad features arguments");
     isti_status st = validate(r, info, (size_t)info->width * 4);
     if (st != ISTI_OK) return st;
     if (info->mode == ISTI_MODE_TIMEWARP || info->mode == ISTI_MODE_BACKWARD_BIDIRECTIONAL)
@@ -722,15 +722,17 @@ isti_status isti_learned_features(isti_renderer *r, const isti_render_info *info
     r->prepared_valid = false;
     st = splat_stage(r, P, s, splat_binned_for(r, P));
     if (st != ISTI_OK) return st;
-    for (int sl = 0; sl < 2; sl++) fa.v[sl] = sl == 0 ? P.ra.fwd[0] : P.ra.bwd[0];
-    fa.x = x;
-    fa.w = info->width;
-    fa.h = info->height;
-    fa.net_w = net_w;
-    fa.net_h = net_h;
-    fa.alpha = info->alpha;
-    e = launch_features(fa, s);
-    if (e != hipSuccess) return dev_fail(r, e, "features launch");
+    if (x) {  // no x: the splat stage alone (a network output of an earlier frame is reused)
+        for (int sl = 0; sl < 2; sl++) fa.v[sl] = sl == 0 ? P.ra.fwd[0] : P.ra.bwd[0];
+        fa.x = x;
+        fa.w = info->width;
+        fa.h = info->height;
+        fa.net_w = net_w;
+        fa.net_h = net_h;
+        fa.alpha = info->alpha;
+        e = launch_features(fa, s);
+        if (e != hipSuccess) return dev_fail(r, e, "features launch");
+    }
     r->prepared = *info;
     r->prepared_valid = true;
     return ISTI_OK;

@@ -1,19 +1,26 @@
 // unet.hip - the learned search initialiser's network on gfx950 (see
 // include/isti_learned.h for the architecture and the C ABI).
 //
-// Convolutions are implicit GEMMs on MFMA (v_mfma_f32_16x16x16_bf16):
-//   forward / input gradient: M = 16 pixels of a row, N = output channels,
-//     K = (ky, kx, input channel); a wave owns 16 pixels x all outputs;
+// Convolutions are implicit GEMMs on MFMA with fp32 accumulation:
+//   forward 3x3: conv3_mfma_kernel on v_mfma_f32_16x16x32_bf16 (M = 16
+//     output channels, N = 16 pixels of a row, K = (chunk, tap, channel)),
+//     the 1x1 head fused into the last 3x3 layer's epilogue;
+//   input gradient: conv3_lds_kernel / conv_kernel on
+//     v_mfma_f32_16x16x16_bf16 (M = 16 pixels, N = output channels);
 //   weight gradient: M = 16 output channels, N = 16 (ky, kx, cin) columns,
 //     K = pixels; partial sums of a wave are added atomically.
 // A 16x16x16 MFMA takes lane l's A[l%16][4(l/16)..+3] and B[4(l/16)..+3][l%16]
-// and returns D[4(l/16)+i][l%16] in element i.
+// and returns D[4(l/16)+i][l%16] in element i; the 16x16x32 form takes eight
+// K values per lane, A[l%16][8(l/16)..+7] and B[8(l/16)..+7][l%16].
 // Activations are NHWC bf16 with channel counts that are multiples of 16.
-// The input of a decoder convolution is concat(upsample2x(low), skip), read
-// on the fly (no concatenated or upsampled tensor is materialised).
+// The input of a decoder convolution is concat(upsample2x(low), skip), and
+// the input of an encoder level's first convolution the 2x2 average of the
+// level above, both read on the fly (nothing pooled, upsampled or
+// concatenated is materialised).
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -42,12 +49,20 @@ __host__ __device__ inline float bf2f(uint16_t h) {
 }
 
 // Input of a convolution: channels [0, ca) from tensor a (at half resolution,
-// read at (y/2, x/2), when up_a), channels [ca, ca+cb) from tensor b.
+// read at (y/2, x/2), when up_a), channels [ca, ca+cb) from tensor b. With
+// pool_a, tensor a is at twice the resolution and the input is its 2x2
+// average (the encoder's pooling, computed where it is read instead of
+// materialised): bf16(((a00 + a01) + (a10 + a11)) * 0.25) in fp32.
 struct Src {
     const uint16_t *a;
     const uint16_t *b;
     int ca, cb, up_a;
+    int pool_a;
 };
+
+__device__ __forceinline__ float pool4(float a00, float a01, float a10, float a11) {
+    return ((a00 + a01) + (a10 + a11)) * 0.25f;
+}
 
 __device__ __forceinline__ bf16x4 load_src4(const Src &s, int n, int y, int x, int H, int W, int ch) {
     const uint16_t *p;
@@ -71,6 +86,11 @@ __device__ __forceinline__ bf16x4 load_src4(const Src &s, int n, int y, int x, i
 }
 
 __device__ __forceinline__ uint16_t load_src1(const Src &s, int n, int y, int x, int H, int W, int ch) {
+    if (s.pool_a) {
+        const size_t W2 = (size_t)W * 2, r = ((size_t)n * H * 2 + 2 * y) * W2 + 2 * x;
+        const uint16_t *p = s.a + r * s.ca + ch;
+        return f2bf(pool4(bf2f(p[0]), bf2f(p[s.ca]), bf2f(p[W2 * s.ca]), bf2f(p[W2 * s.ca + s.ca])));
+    }
     if (ch < s.ca) {
         if (s.up_a) {
             const int h2 = H >> 1, w2 = W >> 1;
@@ -275,6 +295,269 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(Src s, const uint16_t *_
     }
 }
 
+// ---------------------------------------------------------------------------
+// Forward 3x3 convolution on the gfx950 MFMA v_mfma_f32_16x16x32_bf16.
+//
+// The GEMM is laid out transposed: M = 16 output channels (A = weights,
+// lane l holds w[co = l%16][8(l/16) .. +7] of the K step), N = 16 pixels of
+// a row (B = input, lane l holds pixel l%16's channels 8(l/16) .. +7), so a
+// lane's result D[4(l/16) + i][l%16] is four consecutive output channels of
+// one pixel (one 8-byte bf16 store). A block computes 4 rows x 16*MT pixels
+// x 16*NT output channels (one row per wave) and walks the input channels
+// in chunks of KC: each chunk's 6-row input halo and its 9 x 16*NT x KC
+// weight slice are staged in LDS (rows padded by 16 bytes), the next chunk
+// loaded into registers while the current one is multiplied. Channels past
+// cin (layers of 16 or 48 channels) are staged as zeros. The input may be
+// pooled (Src::pool_a), upsampled and concatenated on the fly.
+// With HEAD (NT == 1, cout == 16) the 1x1 output head is applied in the
+// epilogue: the bf16 activation of a lane (channels 4(l/16) .. +3 of pixel
+// l%16) is exactly the B operand of a 16x16x16 MFMA, so the head's 16 x 16
+// weights multiply it without another pass over memory, and the fp32 head
+// output goes to out_f (the activation itself to out_bf, when non-null).
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+__device__ __forceinline__ uint32_t pack_bf2(float lo, float hi) {
+    return (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
+}
+
+// Channels [ch, ch + 8) of input pixel (y, x) of a convolution (ch a
+// multiple of 8; channel counts are multiples of 16, so an 8-channel unit
+// never straddles the concatenation).
+__device__ __forceinline__ uint4 load_src8(const Src &s, int n, int y, int x, int H, int W, int ch) {
+    if (s.pool_a) {
+        const size_t W2 = (size_t)W * 2, r = ((size_t)n * H * 2 + 2 * y) * W2 + 2 * x;
+        const uint16_t *p = s.a + r * s.ca + ch;
+        const uint4 q00 = *reinterpret_cast<const uint4 *>(p), q01 = *reinterpret_cast<const uint4 *>(p + s.ca);
+        const uint4 q10 = *reinterpret_cast<const uint4 *>(p + W2 * s.ca);
+        const uint4 q11 = *reinterpret_cast<const uint4 *>(p + W2 * s.ca + s.ca);
+        const uint32_t a00[4] = {q00.x, q00.y, q00.z, q00.w}, a01[4] = {q01.x, q01.y, q01.z, q01.w};
+        const uint32_t a10[4] = {q10.x, q10.y, q10.z, q10.w}, a11[4] = {q11.x, q11.y, q11.z, q11.w};
+        uint32_t o[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const float lo = pool4(bf2f((uint16_t)a00[j]), bf2f((uint16_t)a01[j]), bf2f((uint16_t)a10[j]),
+                                   bf2f((uint16_t)a11[j]));
+            const float hi = pool4(bf2f((uint16_t)(a00[j] >> 16)), bf2f((uint16_t)(a01[j] >> 16)),
+                                   bf2f((uint16_t)(a10[j] >> 16)), bf2f((uint16_t)(a11[j] >> 16)));
+            o[j] = pack_bf2(lo, hi);
+        }
+        return make_uint4(o[0], o[1], o[2], o[3]);
+    }
+    const uint16_t *p;
+    if (ch < s.ca) {
+        if (s.up_a) p = s.a + (((size_t)n * (H >> 1) + (y >> 1)) * (W >> 1) + (x >> 1)) * s.ca + ch;
+        else p = s.a + (((size_t)n * H + y) * W + x) * s.ca + ch;
+    } else {
+        p = s.b + (((size_t)n * H + y) * W + x) * s.cb + (ch - s.ca);
+    }
+    return *reinterpret_cast<const uint4 *>(p);
+}
+
+// C8 > 0: the layer has cin = 8 * C8 channels with cin % 32 != 0 (16 or 48),
+// staged whole, and the K steps run over the flattened (tap, 8-channel unit)
+// index, four units per step (lane group g takes unit 4 * step + g), so only
+// the last step multiplies padding. C8 == 0: chunks of KC (a multiple of 32)
+// channels, one K step per tap and 32 channels.
+// A block walks the tiles b, b + G, b + 2G, ... of the grid's x dimension
+// (G = gridDim.x) and, per tile, its channel chunks; the next (tile, chunk)
+// is loaded into registers while the current one is multiplied, so a layer
+// of one chunk still overlaps its loads with its MFMAs.
+template <int NT, int MT, int KC, int C8, bool HEAD>
+__global__ __launch_bounds__(256) void conv3_mfma_kernel(Src s, const uint16_t *__restrict__ w,
+                                                         const float *__restrict__ bias, int cin, int N, int H, int W,
+                                                         int cout, uint16_t *out_bf, float *out_f, int relu,
+                                                         const uint16_t *__restrict__ head_w,
+                                                         const float *__restrict__ head_b) {
+    static_assert((C8 > 0 || KC % 32 == 0) && (!HEAD || NT == 1), "conv3_mfma_kernel shape");
+    constexpr int PXW = 16 * MT, COLS = PXW + 2, U = C8 > 0 ? C8 : KC / 8, RU = U + 1;  // RU: 16-byte units per staged row
+    constexpr int CO = 16 * NT;
+    constexpr int IN_UNITS = 6 * COLS * U, W_UNITS = 9 * CO * U;
+    constexpr int IN_PER = (IN_UNITS + 255) / 256, W_PER = (W_UNITS + 255) / 256;
+    __shared__ uint4 s_in[6 * COLS * RU];
+    __shared__ uint4 s_w[9 * CO * RU];
+    const int lane = (int)threadIdx.x & 63, wv = (int)(threadIdx.x >> 6);
+    const int px = lane & 15, g = lane >> 4;
+    const int chunks_x = (W + PXW - 1) / PXW, rgroups = (H + 3) / 4;
+    const int ntiles = N * rgroups * chunks_x;
+    const int nck = C8 > 0 ? 1 : (cin + KC - 1) / KC;
+    const int co_base = (int)blockIdx.y * CO;
+    const int b = (int)blockIdx.x, G = (int)gridDim.x;
+    const int items = b < ntiles ? ((ntiles - 1 - b) / G + 1) * nck : 0;
+    if (items == 0) return;  // uniform over the block
+    auto origin = [&](int q, int &n, int &x0, int &y0) {
+        const int tile = b + (q / nck) * G;
+        n = tile / (rgroups * chunks_x);
+        const int rem = tile - n * rgroups * chunks_x, ry = rem / chunks_x;
+        x0 = (rem - ry * chunks_x) * PXW;
+        y0 = ry * 4;
+    };
+    uint4 pin[IN_PER], pw[W_PER];
+    // the weight slice depends on the chunk only: with one chunk it is staged once
+    auto load_item = [&](int q) {
+        int n, x0, y0;
+        origin(q, n, x0, y0);
+        const int c0 = (q % nck) * KC;
+#pragma unroll
+        for (int k = 0; k < IN_PER; k++) {
+            const int i = (int)threadIdx.x + k * 256;
+            pin[k] = make_uint4(0u, 0u, 0u, 0u);
+            if (i < IN_UNITS) {
+                const int r = i / (COLS * U), rc = i - r * (COLS * U), col = rc / U, u = rc - col * U;
+                const int yy = y0 - 1 + r, xx = x0 - 1 + col, ch = c0 + 8 * u;
+                if (yy >= 0 && yy < H && xx >= 0 && xx < W && ch < cin) pin[k] = load_src8(s, n, yy, xx, H, W, ch);
+            }
+        }
+        if (nck == 1 && q > 0) return;
+#pragma unroll
+        for (int k = 0; k < W_PER; k++) {
+            const int i = (int)threadIdx.x + k * 256;
+            pw[k] = make_uint4(0u, 0u, 0u, 0u);
+            if (i < W_UNITS) {
+                const int t = i / (CO * U), rc = i - t * (CO * U), co = rc / U, u = rc - co * U, ch = c0 + 8 * u;
+                if (ch < cin) pw[k] = *reinterpret_cast<const uint4 *>(w + ((size_t)(co_base + co) * 9 + t) * cin + ch);
+            }
+        }
+    };
+    auto store_item = [&](int q) {
+#pragma unroll
+        for (int k = 0; k < IN_PER; k++) {
+            const int i = (int)threadIdx.x + k * 256;
+            if (i < IN_UNITS) {
+                const int r = i / (COLS * U), rc = i - r * (COLS * U), col = rc / U, u = rc - col * U;
+                s_in[(r * COLS + col) * RU + u] = pin[k];
+            }
+        }
+        if (nck == 1 && q > 0) return;
+#pragma unroll
+        for (int k = 0; k < W_PER; k++) {
+            const int i = (int)threadIdx.x + k * 256;
+            if (i < W_UNITS) {
+                const int t = i / (CO * U), rc = i - t * (CO * U), co = rc / U, u = rc - co * U;
+                s_w[(t * CO + co) * RU + u] = pw[k];
+            }
+        }
+    };
+    bf16x4 hw = bf16x4{0, 0, 0, 0};
+    f32x4 hb = f32x4{0.f, 0.f, 0.f, 0.f};
+    if (HEAD) {  // head weights [co' = l%16][ci = 4(l/16) .. +3] and bias of channels 4(l/16) .. +3
+        const uint2 v = *reinterpret_cast<const uint2 *>(head_w + px * 16 + 4 * g);
+        hw[0] = (short)(v.x & 0xffffu);
+        hw[1] = (short)(v.x >> 16);
+        hw[2] = (short)(v.y & 0xffffu);
+        hw[3] = (short)(v.y >> 16);
+        hb = *reinterpret_cast<const f32x4 *>(head_b + 4 * g);
+    }
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; m++)
+#pragma unroll
+        for (int t = 0; t < NT; t++) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    load_item(0);
+    store_item(0);
+    __syncthreads();
+    for (int q = 0; q < items; q++) {
+        const bool more = q + 1 < items;
+        if (more) load_item(q + 1);  // in flight during this item's MFMAs
+        if constexpr (C8 > 0) {
+            constexpr int STEPS = (9 * C8 + 3) / 4;
+#pragma unroll
+            for (int st = 0; st < STEPS; st++) {
+                const int j = 4 * st + g;
+                const bool valid = j < 9 * C8;
+                const int jj = valid ? j : 0;
+                const int tap = jj / C8, c8 = jj - tap * C8, ky = tap / 3, kx = tap - ky * 3;
+                bf16x8 bv[MT];
+#pragma unroll
+                for (int m = 0; m < MT; m++) bv[m] = as_bf16x8(s_in[((wv + ky) * COLS + m * 16 + px + kx) * RU + c8]);
+#pragma unroll
+                for (int t = 0; t < NT; t++) {
+                    uint4 a = s_w[(tap * CO + t * 16 + px) * RU + c8];
+                    if (!valid) a = make_uint4(0u, 0u, 0u, 0u);  // past the last unit: contributes zero
+                    const bf16x8 av = as_bf16x8(a);
+#pragma unroll
+                    for (int m = 0; m < MT; m++)
+                        acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv[m], acc[m][t], 0, 0, 0);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int ky = 0; ky < 3; ky++) {
+#pragma unroll
+                for (int kx = 0; kx < 3; kx++) {
+#pragma unroll
+                    for (int u2 = 0; u2 < KC / 32; u2++) {
+                        bf16x8 bv[MT];
+#pragma unroll
+                        for (int m = 0; m < MT; m++)
+                            bv[m] = as_bf16x8(s_in[((wv + ky) * COLS + m * 16 + px + kx) * RU + u2 * 4 + g]);
+#pragma unroll
+                        for (int t = 0; t < NT; t++) {
+                            const bf16x8 av = as_bf16x8(s_w[((ky * 3 + kx) * CO + t * 16 + px) * RU + u2 * 4 + g]);
+#pragma unroll
+                            for (int m = 0; m < MT; m++)
+                                acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv[m], acc[m][t], 0, 0, 0);
+                        }
+                    }
+                }
+            }
+        }
+        if (q % nck == nck - 1) {  // the tile's last chunk: bias, activation, stores
+            int n, x0, y0;
+            origin(q, n, x0, y0);
+            const int y = y0 + wv;
+            if (y < H) {
+#pragma unroll
+                for (int t = 0; t < NT; t++) {
+                    const int co = co_base + t * 16 + 4 * g;
+                    const f32x4 bsv = *reinterpret_cast<const f32x4 *>(bias + co);
+#pragma unroll
+                    for (int m = 0; m < MT; m++) {
+                        const int x = x0 + m * 16 + px;
+                        f32x4 v;
+#pragma unroll
+                        for (int i = 0; i < 4; i++) {
+                            v[i] = acc[m][t][i] + bsv[i];
+                            if (relu) v[i] = v[i] > 0.0f ? v[i] : 0.0f;
+                        }
+                        const size_t o = (((size_t)n * H + y) * W + x) * cout + co;
+                        const uint2 packed = make_uint2(pack_bf2(v[0], v[1]), pack_bf2(v[2], v[3]));
+                        if (HEAD) {
+                            // every lane takes part in the head's MFMA; lanes past the row's end only compute
+                            bf16x4 av;
+                            av[0] = (short)(packed.x & 0xffffu);
+                            av[1] = (short)(packed.x >> 16);
+                            av[2] = (short)(packed.y & 0xffffu);
+                            av[3] = (short)(packed.y >> 16);
+                            const f32x4 h =
+                                __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(hw, av, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+                            if (x < W) {
+                                if (out_bf) *reinterpret_cast<uint2 *>(out_bf + o) = packed;
+                                *reinterpret_cast<f32x4 *>(out_f + (((size_t)n * H + y) * W + x) * 16 + 4 * g) =
+                                    f32x4{h[0] + hb[0], h[1] + hb[1], h[2] + hb[2], h[3] + hb[3]};
+                            }
+                        } else if (x < W) {
+                            if (out_f) *reinterpret_cast<f32x4 *>(out_f + o) = v;
+                            else *reinterpret_cast<uint2 *>(out_bf + o) = packed;
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < MT; m++)
+#pragma unroll
+                for (int t = 0; t < NT; t++) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+        __syncthreads();  // the next item overwrites the staged one
+        if (more) {
+            store_item(q + 1);
+            __syncthreads();
+        }
+    }
+}
+
 // dw[co][kk] += sum_p g[p][co] * in[p + off(kk)][ci(kk)], kk = (ky*ks+kx)*cin+ci
 __global__ __launch_bounds__(256) void wgrad_kernel(Src s, const uint16_t *__restrict__ g, int cout, int cin, int ks,
                                                     int N, int H, int W, float *dw) {
@@ -327,21 +610,6 @@ __global__ void mask_kernel(const uint16_t *__restrict__ y, const float *__restr
     if (i >= n) return;
     const float d = dy[i];
     g[i] = (!relu || bf2f(y[i]) > 0.0f) ? f2bf(d) : (uint16_t)0;
-}
-
-__global__ void pool_fwd_kernel(const uint16_t *__restrict__ x, uint16_t *y, int N, int H, int W, int C) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int h2 = H >> 1, w2 = W >> 1;
-    if (i >= (long)N * h2 * w2 * C) return;
-    const int c = (int)(i % C);
-    long r = i / C;
-    const int xo = (int)(r % w2);
-    r /= w2;
-    const int yo = (int)(r % h2);
-    const int n = (int)(r / h2);
-    auto at = [&](int yy, int xx) { return bf2f(x[(((size_t)n * H + yy) * W + xx) * C + c]); };
-    const float s = (at(2 * yo, 2 * xo) + at(2 * yo, 2 * xo + 1)) + (at(2 * yo + 1, 2 * xo) + at(2 * yo + 1, 2 * xo + 1));
-    y[i] = f2bf(s * 0.25f);
 }
 
 // dx[n][y][x][c] += dy[n][y/2][x/2][c] / 4
@@ -446,9 +714,7 @@ struct isti_unet {
     size_t w_off[NL], b_off[NL];
     uint16_t *wb[NL] = {}, *wt[NL] = {};
     uint16_t *y[NL] = {};     // post-activation outputs (bf16); the head writes fp32 to the caller
-    uint16_t *pool[4] = {};   // pooled inputs of levels 1..3
     float *dy[NL] = {};       // gradient of each layer's output
-    float *dpool[4] = {};     // gradient of pooled tensors
     float *dcat = nullptr;    // gradient of a layer input (largest size)
     uint16_t *gm = nullptr;   // masked bf16 output gradient (largest size)
     const uint16_t *x = nullptr;
@@ -528,14 +794,90 @@ hipError_t launch_conv(const Src &s, const uint16_t *w, const float *bias, int c
     }
 }
 
+// Forward 3x3 convolutions (conv3_mfma_kernel). Inputs of 16 or 48
+// channels are staged whole with flattened K steps; others in chunks of 64
+// channels (more than 32 channels) or 32. The chunking fixes the K order of
+// the sums; the wave shape (NT, MT) and the grid do not, so every shape of a
+// layer gives the same bits. Shape: the most output tiles per wave whose
+// grid still has CONV3_MIN_BLOCKS tiles (8 per CU), else the most tiles (the
+// deep levels are small: 60 x 34 pixels at 1080p); at most CONV3_GRID
+// blocks walk them (measured on the 480 x 272 network: 8 tiles per CU and
+// 4 blocks per CU beat 4 and 2, 0.171 -> 0.156 ms per forward pass).
+constexpr long CONV3_MIN_BLOCKS = 2048;
+constexpr long CONV3_GRID = 1024;
+
+template <int NT, int MT, int KC, int C8, bool HEAD>
+hipError_t launch_conv3_shape(const Src &s, const uint16_t *w, const float *bias, int cin, int cout, int N, int H, int W,
+                              uint16_t *out_bf, float *out_f, int relu, const uint16_t *head_w, const float *head_b,
+                              hipStream_t st) {
+    const long tiles = (long)N * ((H + 3) / 4) * ((W + 16 * MT - 1) / (16 * MT));
+    const dim3 grid((unsigned)std::min(tiles, CONV3_GRID), (unsigned)(cout / (16 * NT))), block(256);
+    hipLaunchKernelGGL((conv3_mfma_kernel<NT, MT, KC, C8, HEAD>), grid, block, 0, st, s, w, bias, cin, N, H, W, cout,
+                       out_bf, out_f, relu, head_w, head_b);
+    return hipGetLastError();
+}
+
+template <int KC, int C8>
+hipError_t launch_conv3_kc(int nt, int mt, const Src &s, const uint16_t *w, const float *bias, int cin, int cout, int N,
+                           int H, int W, uint16_t *out_bf, float *out_f, int relu, hipStream_t st) {
+    switch (nt * 8 + mt) {
+        case 1 * 8 + 4: return launch_conv3_shape<1, 4, KC, C8, false>(s, w, bias, cin, cout, N, H, W, out_bf, out_f, relu, nullptr, nullptr, st);
+        case 1 * 8 + 2: return launch_conv3_shape<1, 2, KC, C8, false>(s, w, bias, cin, cout, N, H, W, out_bf, out_f, relu, nullptr, nullptr, st);
+        case 2 * 8 + 2: return launch_conv3_shape<2, 2, KC, C8, false>(s, w, bias, cin, cout, N, H, W, out_bf, out_f, relu, nullptr, nullptr, st);
+        case 2 * 8 + 1: return launch_conv3_shape<2, 1, KC, C8, false>(s, w, bias, cin, cout, N, H, W, out_bf, out_f, relu, nullptr, nullptr, st);
+        case 4 * 8 + 1: return launch_conv3_shape<4, 1, KC, C8, false>(s, w, bias, cin, cout, N, H, W, out_bf, out_f, relu, nullptr, nullptr, st);
+        default: return launch_conv3_shape<1, 1, KC, C8, false>(s, w, bias, cin, cout, N, H, W, out_bf, out_f, relu, nullptr, nullptr, st);
+    }
+}
+
+void conv3_shape(int cout, int N, int H, int W, int *nt, int *mt) {
+    static const int shapes[][2] = {{1, 4}, {2, 2}, {4, 1}, {1, 2}, {2, 1}, {1, 1}};
+    int most = 5;
+    long most_blocks = -1;
+    for (int i = 0; i < 6; i++) {
+        const int t = shapes[i][0], m = shapes[i][1];
+        if ((cout / 16) % t != 0) continue;
+        const long blocks = (long)N * ((H + 3) / 4) * ((W + 16 * m - 1) / (16 * m)) * (cout / (16 * t));
+        if (blocks >= CONV3_MIN_BLOCKS) {
+            *nt = t, *mt = m;
+            return;
+        }
+        if (blocks > most_blocks) most_blocks = blocks, most = i;
+    }
+    *nt = shapes[most][0], *mt = shapes[most][1];
+}
+
+hipError_t launch_conv3_fwd(const Src &s, const uint16_t *w, const float *bias, int cin, int cout, int N, int H, int W,
+                            uint16_t *out_bf, float *out_f, int relu, hipStream_t st) {
+    if (cout % 16 != 0 || cin % 16 != 0) return hipErrorInvalidValue;
+    int nt, mt;
+    conv3_shape(cout, N, H, W, &nt, &mt);
+    if (cin == 16) return launch_conv3_kc<16, 2>(nt, mt, s, w, bias, cin, cout, N, H, W, out_bf, out_f, relu, st);
+    if (cin == 48) return launch_conv3_kc<48, 6>(nt, mt, s, w, bias, cin, cout, N, H, W, out_bf, out_f, relu, st);
+    return cin > 32 ? launch_conv3_kc<64, 0>(nt, mt, s, w, bias, cin, cout, N, H, W, out_bf, out_f, relu, st)
+                    : launch_conv3_kc<32, 0>(nt, mt, s, w, bias, cin, cout, N, H, W, out_bf, out_f, relu, st);
+}
+
+// The last 3x3 convolution (16 input and output channels) with the 1x1 head
+// in its epilogue: y (bf16, when non-null) and the head's fp32 output.
+hipError_t launch_conv3_head(const Src &s, const uint16_t *w, const float *bias, int cin, int N, int H, int W,
+                             uint16_t *y, float *out, const uint16_t *head_w, const float *head_b, hipStream_t st) {
+    if (cin != 16) return hipErrorInvalidValue;
+    int nt, mt;
+    conv3_shape(16, N, H, W, &nt, &mt);
+    if (mt == 4) return launch_conv3_shape<1, 4, 16, 2, true>(s, w, bias, cin, 16, N, H, W, y, out, 1, head_w, head_b, st);
+    if (mt == 2) return launch_conv3_shape<1, 2, 16, 2, true>(s, w, bias, cin, 16, N, H, W, y, out, 1, head_w, head_b, st);
+    return launch_conv3_shape<1, 1, 16, 2, true>(s, w, bias, cin, 16, N, H, W, y, out, 1, head_w, head_b, st);
+}
+
 // input of layer l in the forward pass
 Src layer_src(const isti_unet *n, int l) {
     Src s{nullptr, nullptr, LAYERS[l].ca, LAYERS[l].cb, LAYERS[l].up_a};
     switch (l) {
         case 0: s.a = n->x; break;
-        case 2: s.a = n->pool[1]; break;
-        case 4: s.a = n->pool[2]; break;
-        case 6: s.a = n->pool[3]; break;
+        case 2:
+        case 4:
+        case 6: s.a = n->y[l - 1]; s.pool_a = 1; break;  // 2x2 average of the level above
         case 8: s.a = n->y[7]; s.b = n->y[5]; break;   // up(e3), e2
         case 10: s.a = n->y[9]; s.b = n->y[3]; break;  // up(d2), e1
         case 12: s.a = n->y[11]; s.b = n->y[1]; break; // up(d1), e0
@@ -598,11 +940,6 @@ isti_status isti_unet_create(int32_t device, uint32_t width, uint32_t height, in
         if (e == hipSuccess && l < NL - 1) e = dalloc(n, (void **)&n->dy[l], px * LAYERS[l].cout * 4);
         max_cat = std::max(max_cat, px * (size_t)(LAYERS[l].ca + LAYERS[l].cb));
         max_out = std::max(max_out, px * (size_t)LAYERS[l].cout);
-    }
-    for (int lv = 1; lv <= 3 && e == hipSuccess; lv++) {
-        const int c = LAYERS[2 * lv].ca;
-        e = dalloc(n, (void **)&n->pool[lv], px_at(n, lv) * c * 2);
-        if (e == hipSuccess) e = dalloc(n, (void **)&n->dpool[lv], px_at(n, lv) * c * 4);
     }
     if (e == hipSuccess) e = dalloc(n, (void **)&n->dcat, max_cat * 4);
     if (e == hipSuccess) e = dalloc(n, (void **)&n->gm, max_out * 2);
@@ -683,18 +1020,18 @@ isti_status isti_unet_forward(isti_unet *n, const uint16_t *x, float *out, void 
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     hipError_t e = hipSetDevice(n->device);
     n->x = x;
-    for (int l = 0; l < NL && e == hipSuccess; l++) {
+    // the 3x3 layers; the last one carries the 1x1 head in its epilogue
+    for (int l = 0; l < NL - 1 && e == hipSuccess; l++) {
         const int lv = LEVEL[l];
         const int H = n->H >> lv, W = n->W >> lv;
-        if (l == 2 || l == 4 || l == 6) {  // pool the previous level's output
-            const int c = LAYERS[l].ca;
-            const long cnt = (long)px_at(n, lv) * c;
-            hipLaunchKernelGGL(pool_fwd_kernel, dim3(blocks_for(cnt, 0)), dim3(256), 0, st, n->y[l - 1], n->pool[lv],
-                               n->N, H * 2, W * 2, c);
-        }
         const Src s = layer_src(n, l);
-        e = launch_conv(s, n->wb[l], n->p + n->b_off[l], LAYERS[l].ca + LAYERS[l].cb, LAYERS[l].cout, LAYERS[l].ks,
-                        n->N, H, W, l < NL - 1 ? n->y[l] : nullptr, l < NL - 1 ? nullptr : out, LAYERS[l].relu, st);
+        const int cin = LAYERS[l].ca + LAYERS[l].cb;
+        if (l == NL - 2)
+            e = launch_conv3_head(s, n->wb[l], n->p + n->b_off[l], cin, n->N, H, W, n->y[l], out, n->wb[NL - 1],
+                                  n->p + n->b_off[NL - 1], st);
+        else
+            e = launch_conv3_fwd(s, n->wb[l], n->p + n->b_off[l], cin, LAYERS[l].cout, n->N, H, W, n->y[l], nullptr,
+                                 LAYERS[l].relu, st);
     }
     return e == hipSuccess ? ISTI_OK : fail(e, "forward");
 }
@@ -705,8 +1042,6 @@ isti_status isti_unet_backward(isti_unet *n, const float *dout, void *stream) {
     hipError_t e = hipSetDevice(n->device);
     for (int l = 0; l < NL - 1 && e == hipSuccess; l++)
         e = hipMemsetAsync(n->dy[l], 0, px_at(n, LEVEL[l]) * LAYERS[l].cout * 4, st);
-    for (int lv = 1; lv <= 3 && e == hipSuccess; lv++)
-        e = hipMemsetAsync(n->dpool[lv], 0, px_at(n, lv) * LAYERS[2 * lv].ca * 4, st);
     for (int l = NL - 1; l >= 0 && e == hipSuccess; l--) {
         const LayerDef &L = LAYERS[l];
         const int lv = LEVEL[l], H = n->H >> lv, W = n->W >> lv, cin = L.ca + L.cb;

@@ -35,6 +35,15 @@ LAYERS = [
     (32, 16, 16, 3, 1, 1), (16, 0, 16, 3, 0, 1), (16, 0, 16, 1, 0, 0),
 ]
 
+LEVEL = [0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0, 0]  # resolution level of each layer (size / 2^level)
+
+
+def flop_per_cell() -> float:
+    """Forward FLOPs per network cell (one multiply-add = 2), from LAYERS:
+    each layer's 2 * cout * k^2 * cin at its level's share of the cells."""
+    return float(sum(2 * co * ks * ks * (ca + cb) / 4 ** lv for (ca, cb, co, ks, _, _), lv in zip(LAYERS, LEVEL)))
+
+
 _bound = False
 
 
@@ -148,13 +157,19 @@ def set_search_capture(renderer, q_out) -> None:
     _ok(_lib().isti_set_search_capture(renderer._h, q_out.data_ptr() if q_out is not None else None), "capture")
 
 
-def features(renderer, info, x, stream=None) -> None:
+def features(renderer, info, x, stream=None, net_size=None) -> None:
     """Splat stage of frame `info` plus the network input x (int16 bf16 bits,
-    [2, net_h, net_w, 16]); prepares the frame for loss() and render_learned()."""
+    [2, net_h, net_w, 16]); prepares the frame for loss() and render_learned().
+    x None runs the splat stage alone, for a frame that reuses the network
+    output of an earlier frame of its keyframe pair (net_size = (net_w, net_h))."""
     ci = info.to_c()
-    n_, nh, nw, _ = x.shape
-    N.check(_lib().isti_learned_features(renderer._h, ctypes.byref(ci), x.data_ptr(), nw, nh, _stream(stream)),
-            renderer._h)
+    if x is None:
+        nw, nh = net_size
+        ptr = None
+    else:
+        n_, nh, nw, _ = x.shape
+        ptr = x.data_ptr()
+    N.check(_lib().isti_learned_features(renderer._h, ctypes.byref(ci), ptr, nw, nh, _stream(stream)), renderer._h)
 
 
 def loss(renderer, out, q_target, dout, loss_acc, conf_weight=0.1, stream=None) -> None:
@@ -227,12 +242,25 @@ class LearnedInit:
         self.stats = torch.zeros(8, dtype=torch.int32, device=dev)
 
     def render(self, renderer, render_buffer, info, conf_threshold: float = 0.5, with_stats: bool = False,
-               stream=None) -> None:
-        """Splat stage + features, network forward, learned-start resolve (stream-ordered)."""
-        features(renderer, info, self.x, stream)
-        self.net.forward(self.x, self.out, stream)
+               stream=None, reuse: bool = False) -> None:
+        """Splat stage + features, network forward, learned-start resolve (stream-ordered).
+
+        reuse=True skips the features and the network and resolves with the
+        network output of the last frame rendered without reuse: the network
+        runs once per keyframe pair, on its first frame (render_pair_frame)."""
+        if reuse:
+            features(renderer, info, None, stream, net_size=(self.net_w, self.net_h))
+        else:
+            features(renderer, info, self.x, stream)
+            self.net.forward(self.x, self.out, stream)
         render_learned(renderer, render_buffer, info, self.out, conf_threshold,
                        self.stats if with_stats else None, stream)
+
+    def render_pair_frame(self, renderer, render_buffer, info, first_of_pair: bool, conf_threshold: float = 0.5,
+                          with_stats: bool = False, stream=None) -> None:
+        """The per-pair schedule: the network runs on the first frame of a
+        keyframe pair and its output serves the pair's later frames."""
+        self.render(renderer, render_buffer, info, conf_threshold, with_stats, stream, reuse=not first_of_pair)
 
     def read_stats(self, reset: bool = True) -> dict:
         v = self.stats.cpu().numpy().astype(np.int64)
@@ -337,12 +365,15 @@ def train(cfg: TrainConfig = TrainConfig(), init_params: np.ndarray = None, log=
 # Evaluation against the plain search (same kernel, learned starts disabled)
 # ---------------------------------------------------------------------------
 def evaluate(params: np.ndarray, width: int = 1920, height: int = 1080, shift: int = 2, workload: int = 1,
-             seeds=(900001, 900002), pairs_per_seed: int = 2, conf_threshold: float = 0.5, max_iter: int = 32):
+             seeds=(900001, 900002), pairs_per_seed: int = 2, conf_threshold: float = 0.5, max_iter: int = 32,
+             per_pair: bool = False):
     """Iterations saved, fallback rate and PSNR of learned starts vs the plain search.
 
     PSNR (RGB, the reference's scripts/eval.py:57,68) is measured against the
     generator's ground-truth render of each target frame, over pixels both
     methods wrote; `psnr_learned_vs_plain` compares the two outputs directly.
+    per_pair: the network runs on the first frame of each keyframe pair only
+    and the pair's later frames reuse its output.
     """
     import torch
 
@@ -367,9 +398,11 @@ def evaluate(params: np.ndarray, width: int = 1920, height: int = 1080, shift: i
                 r.set_frame(fd)
                 plain_buf.zero_()
                 learn_buf.zero_()
-                li.render(r, plain_buf, info, conf_threshold=1.0, with_stats=True)  # plain starts, counted
+                # plain starts, counted (threshold 1 reads no network output)
+                li.render(r, plain_buf, info, conf_threshold=1.0, with_stats=True, reuse=True)
                 sp = li.read_stats()
-                li.render(r, learn_buf, info, conf_threshold=conf_threshold, with_stats=True)
+                li.render(r, learn_buf, info, conf_threshold=conf_threshold, with_stats=True,
+                          reuse=per_pair and b > 1)
                 sl = li.read_stats()
                 for key in tot_p:
                     tot_p[key] += sp[key]
@@ -408,6 +441,34 @@ def evaluate(params: np.ndarray, width: int = 1920, height: int = 1080, shift: i
     return res
 
 
+def time_forward(width: int = 480, height: int = 272, reps: int = 100) -> dict:
+    """Average time of the network's forward pass (both directions, batch 2)
+    at width x height cells, on seeded inputs, after 5 untimed passes."""
+    import time
+
+    import torch
+
+    params = load_params(WEIGHTS)[0] if WEIGHTS.exists() else None
+    net = UNet(width, height, batch=2)
+    if params is not None:
+        net.set_params(params)
+    g = torch.Generator().manual_seed(0)
+    x = torch.randn((2, height, width, CH), generator=g).to(torch.bfloat16).view(torch.int16).cuda()
+    out = torch.zeros((2, height, width, CH), dtype=torch.float32, device="cuda")
+    for _ in range(5):
+        net.forward(x, out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        net.forward(x, out)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    net.close()
+    gflop = flop_per_cell() * 2 * width * height / 1e9
+    return {"width": width, "height": height, "forward_ms": round(ms, 4), "gflop": round(gflop, 2),
+            "tflop_s": round(gflop / ms, 1)}
+
+
 def _main(argv=None):
     import argparse
 
@@ -424,6 +485,11 @@ def _main(argv=None):
     e.add_argument("--shift", type=int, default=2)
     e.add_argument("--conf", type=float, default=0.5)
     e.add_argument("--json", default=None)
+    e.add_argument("--per-pair", action="store_true", help="network once per keyframe pair")
+    f = sub.add_parser("forward", help="time the network's forward pass (shipped weights, seeded input)")
+    f.add_argument("--width", type=int, default=480)
+    f.add_argument("--height", type=int, default=272)
+    f.add_argument("--reps", type=int, default=100)
     a = ap.parse_args(argv)
     if a.cmd == "train":
         cfg = TrainConfig(**{f: getattr(a, f) for f in asdict(TrainConfig())})
@@ -432,9 +498,11 @@ def _main(argv=None):
                 "loss_last100": float(losses[-100:].mean())}
         save_params(a.out, params, meta)
         print(json.dumps(meta), flush=True)
+    elif a.cmd == "forward":
+        print(json.dumps(time_forward(a.width, a.height, a.reps)), flush=True)
     else:
         params, meta = load_params(a.weights)
-        res = evaluate(params, a.width, a.height, a.shift, conf_threshold=a.conf)
+        res = evaluate(params, a.width, a.height, a.shift, conf_threshold=a.conf, per_pair=a.per_pair)
         print(json.dumps(res), flush=True)
         if a.json:
             Path(a.json).write_text(json.dumps(res, indent=1) + "\n")

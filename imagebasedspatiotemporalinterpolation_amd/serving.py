@@ -25,7 +25,10 @@ requests with continuous batching:
   - with search="learned" every request takes the learned-start path
     (splat + features, the network, the learned resolve; learned.py) on its
     stream instead of the batched plain launch, so the two methods can be
-    served side by side on the same mix;
+    served side by side on the same mix; the network runs on the first
+    request of each keyframe pair of a session eye and its output (kept per
+    session eye, counted in the admission footprint) serves the pair's later
+    requests (learned_per_pair=False runs it on every request);
   - latency is measured from arrival to the host observing completion.
 
 Results do not depend on the mix: a request renders exactly what it would
@@ -222,6 +225,16 @@ class Stats:
         }
 
 
+def learned_out_footprint(width: int, height: int, eyes: int = 1) -> int:
+    """Bytes of the network outputs a learned-start session keeps (both
+    directions, 16 fp32 channels per cell, per eye)."""
+    from . import learned as Ln
+
+    k = learned_shift(width, height)
+    nw, nh = Ln.net_grid(width, height, k)
+    return eyes * 2 * nw * nh * 16 * 4
+
+
 def learned_shift(width: int, height: int) -> int:
     """Reduction (as a power of two, at least 2x) from a session's size to
     the learned path's network grid: the smallest that keeps the grid within
@@ -241,12 +254,14 @@ class Server:
 
     def __init__(self, hbm_budget: int, max_batch: int = 32, n_streams: int = 2, graph: bool = True,
                  max_iter: int = -1, store: Optional[KeyframeStore] = None, search: str = "plain",
-                 learned_params=None):
+                 learned_params=None, learned_per_pair: bool = True):
         import torch
 
         if search not in ("plain", "learned"):
             raise ValueError("search is 'plain' or 'learned'")
         self.search = search
+        self.learned_per_pair = learned_per_pair
+        self._net_out: Dict[tuple, tuple] = {}  # (session id, eye) -> (pair, network output)
         self._params = learned_params
         if search == "learned" and learned_params is None:
             from . import learned as Ln
@@ -272,9 +287,17 @@ class Server:
         self._next_sid = 0
 
     # -- sessions ---------------------------------------------------------
+    def session_bytes(self, cls: SessionClass) -> int:
+        """A session's own device bytes: renderer state, outputs and, for
+        learned starts kept per pair, its network outputs."""
+        fp = session_footprint(cls.width, cls.height, cls.eyes)
+        if self.search == "learned" and self.learned_per_pair:
+            fp += learned_out_footprint(cls.width, cls.height, cls.eyes)
+        return fp
+
     def footprint(self, cls: SessionClass) -> int:
         """Bytes admitting a session of `cls` would add now."""
-        fp = session_footprint(cls.width, cls.height, cls.eyes)
+        fp = self.session_bytes(cls)
         if cls.name not in self.resident:
             fp += self.store.footprint(cls)
         return fp
@@ -282,7 +305,7 @@ class Server:
     def open_session(self, cls: SessionClass, pair: int = 0) -> int:
         """Queues a session starting at keyframe pair `pair`; it becomes active
         once its footprint fits the budget."""
-        need = session_footprint(cls.width, cls.height, cls.eyes) + self.store.footprint(cls)
+        need = self.session_bytes(cls) + self.store.footprint(cls)
         if need > self.budget:
             raise ValueError(f"session {cls.name} needs {need} B, more than the whole budget")
         sid = self._next_sid
@@ -299,6 +322,7 @@ class Server:
                 return
             self.waiting.popleft()
             self.sessions[sid] = Session(sid, cls, self.store, pair)
+            self.sessions[sid].footprint = self.session_bytes(cls)
             self.resident.add(cls.name)
             self.used += fp
 
@@ -309,6 +333,8 @@ class Server:
         self.drain()
         s = self.sessions.pop(sid)
         self.used -= s.footprint
+        for e in range(s.cls.eyes):
+            self._net_out.pop((sid, e), None)
         self.pending = collections.deque(r for r in self.pending if r.sid != sid)
         s.close()
         self._admit()
@@ -353,15 +379,15 @@ class Server:
             outs = r.outs if r.outs is not None else s.outs
             for e, rend in enumerate(s.renderers):
                 work.append((rend, s.info(r.pair, r.b, e, self.max_iter), outs[e]))
-                owners.append(s.cls)
+                owners.append((s.cls, r.sid, e, r.pair))
             r.launched = now
             self.busy.add(r.sid)
         import torch
 
         with torch.cuda.stream(self.streams[i]):
             if self.search == "learned":
-                for cls, (rend, info, out) in zip(owners, work):
-                    self.learned_init(cls, i).render(rend, out, info, stream=self.streams[i])
+                for (cls, sid, e, pair), (rend, info, out) in zip(owners, work):
+                    self._render_learned(i, cls, sid, e, pair, rend, info, out)
             else:
                 self.batches[i].render(work, stream=self.streams[i])
             ev = torch.cuda.Event()
@@ -393,6 +419,31 @@ class Server:
         for li in self._learned.values():
             li.close()
         self._learned = {}
+
+    def _render_learned(self, i, cls, sid, eye, pair, rend, info, out):
+        """One eye of a request on stream i through the learned path; with
+        per-pair outputs, the network runs only when the eye's stored output
+        belongs to another pair (a session's requests never overlap, so its
+        output is not in use by another stream)."""
+        import torch
+
+        from . import learned as Ln
+
+        li = self.learned_init(cls, i)
+        st = self.streams[i]
+        if not self.learned_per_pair:
+            li.render(rend, out, info, stream=st)
+            return
+        cached = self._net_out.get((sid, eye))
+        if cached is not None and cached[0] == pair:
+            Ln.features(rend, info, None, st, net_size=(li.net_w, li.net_h))
+            Ln.render_learned(rend, out, info, cached[1], stream=st)
+            return
+        buf = cached[1] if cached is not None else torch.empty_like(li.out)
+        Ln.features(rend, info, li.x, st)
+        li.net.forward(li.x, buf, st)
+        Ln.render_learned(rend, out, info, buf, stream=st)
+        self._net_out[(sid, eye)] = (pair, buf)
 
     def learned_init(self, cls: SessionClass, i: int):
         """The learned-start network state of stream i for sessions of cls."""

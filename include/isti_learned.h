@@ -77,7 +77,12 @@ isti_status isti_set_search_capture(isti_renderer *r, uint32_t *q_out);
  * net_w x net_h (render size / 2^k for k in 0..3, each rounded up to a
  * multiple of 8) from the main keyframes and the fresh cam_ff planes.
  * x: device bf16 [2][net_h][net_w][16]. Displacement features are in
- * network cells (2^k render pixels). */
+ * network cells (2^k render pixels). x NULL runs the splat stage alone: the
+ * frame is prepared for isti_render_learned with a network output computed
+ * for an earlier frame (the network runs once per keyframe pair, on its
+ * first frame, and later frames of the pair reuse its output; the learned
+ * start stays relative to each frame's own one-step start, and the plain
+ * search still judges it). */
 isti_status isti_learned_features(isti_renderer *r, const isti_render_info *info, uint16_t *x, uint32_t net_w,
                                   uint32_t net_h, void *stream);
 

@@ -154,6 +154,48 @@ def test_render_learned_needs_its_features():
     r.close()
 
 
+@pytest.mark.skipif(not Ln.WEIGHTS.exists(), reason="no trained weights in the package")
+def test_network_output_reused_across_a_pair():
+    """The per-pair schedule: a later frame of the pair prepared by the splat
+    stage alone (no features) and resolved with the first frame's network
+    output renders the same bytes as the same frame prepared with its own
+    features and resolved with that output; the reused output keeps at least
+    half as many learned starts as the frame's own output."""
+    params, _ = Ln.load_params(Ln.WEIGHTS)
+    cfg = F.scaled(1, 6, 2)
+    fwd, bwd = F.pair(cfg, 0)
+    fpa = cfg.keyframe_spacing
+    li = Ln.LearnedInit(fwd[0]["z"].shape[1], fwd[0]["z"].shape[0], 1, params)
+    w, h = li.width, li.height
+    infos = []
+    for b in range(1, fpa):
+        vp, ivp, _, _ = F.target(cfg, 0, b)
+        infos.append(R.RenderInfo((w, h), b / fpa, vp, ivp, R.AdditionalRenderInfo(fpa, 0, True, R.RenderMode.OURS),
+                                  32))
+    r, fd, _ = _gpu_setup(fwd, bwd, vp, ivp, w, h, infos[0].alpha, fpa, 32)
+    first = torch.zeros((h, w, 4), dtype=torch.uint8, device="cuda")
+    li.render_pair_frame(r, first, infos[0], True)
+    out1 = li.out.clone()
+    for info in infos[1:]:
+        reused = torch.zeros_like(first)
+        li.render_pair_frame(r, reused, info, False, with_stats=True)
+        st = li.read_stats()
+        own = torch.zeros_like(first)
+        Ln.features(r, info, li.x)  # this frame's features, unused below
+        Ln.render_learned(r, own, info, out1)
+        torch.cuda.synchronize()
+        assert torch.equal(reused, own), f"alpha {info.alpha}"
+        fresh = torch.zeros_like(first)
+        li.render(r, fresh, info, with_stats=True)  # the frame's own network output
+        sf = li.read_stats()
+        print(f"alpha {info.alpha}: kept {st['kept'] / st['searches']:.3f} reused, "
+              f"{sf['kept'] / sf['searches']:.3f} own output")
+        assert st["kept"] >= 0.5 * sf["kept"], (st, sf)  # the pair's output still serves its later frames
+        li.out.copy_(out1)
+    li.close()
+    r.close()
+
+
 def test_short_training_lowers_the_loss_and_checkpoint_round_trip(tmp_path):
     cfg = Ln.TrainConfig(pairs=90, sequences=3, width=320, height=176, lr=2e-3, lr_final=2e-3)
     params, losses = Ln.train(cfg, log=None)

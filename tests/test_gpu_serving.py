@@ -24,16 +24,18 @@ def store():
     return S.KeyframeStore(seed=3, n_keyframes=3)
 
 
-def replay_alone(store, cls, reqs, max_iter=-1, learned=None):
+def replay_alone(store, cls, reqs, max_iter=-1, learned=None, per_pair=True):
     """A session's requests rendered one by one by fresh renderers that are
-    handed each pair with set_frame (through `learned`, a LearnedInit, when
-    given); returns the outputs."""
+    handed each pair with set_frame (through `learned`, one LearnedInit per
+    eye, when given: with per_pair, the network runs on each pair's first
+    request and its output serves the pair's later ones); returns the outputs."""
     cfg = store.config(cls)
     fpa = cfg.keyframe_spacing
     rends = [R.Renderer(0) for _ in range(cls.eyes)]
     cur, outs = None, []
     for pair, b, prev in reqs:
-        if pair != cur:
+        first = pair != cur
+        if first:
             for e, r in enumerate(rends):
                 r.set_frame(store.pair(cls, pair, e))
             cur = pair
@@ -44,7 +46,7 @@ def replay_alone(store, cls, reqs, max_iter=-1, learned=None):
                                 R.AdditionalRenderInfo(fpa, 0, False, R.RenderMode.OURS), max_iter)
             alone = torch.from_numpy(prev[e].copy()).cuda()
             if learned is not None:
-                learned.render(r, alone, info)
+                learned[e].render(r, alone, info, reuse=per_pair and not first)
             else:
                 r.render(alone, info)
             res.append(alone)
@@ -89,10 +91,12 @@ def check_alone(srv, log):
         if srv.search == "learned":
             from imagebasedspatiotemporalinterpolation_amd import learned as Ln
 
-            li = Ln.LearnedInit(s.cls.width, s.cls.height, S.learned_shift(s.cls.width, s.cls.height), srv._params)
-        alone = replay_alone(srv.store, s.cls, [(r.pair, r.b, prev) for r, prev in entries], srv.max_iter, li)
-        if li is not None:
-            li.close()
+            li = [Ln.LearnedInit(s.cls.width, s.cls.height, S.learned_shift(s.cls.width, s.cls.height), srv._params)
+                  for _ in range(s.cls.eyes)]
+        alone = replay_alone(srv.store, s.cls, [(r.pair, r.b, prev) for r, prev in entries], srv.max_iter, li,
+                             srv.learned_per_pair)
+        for x in li or []:
+            x.close()
         for (r, _), a in zip(entries, alone):
             for e in range(s.cls.eyes):
                 np.testing.assert_array_equal(r.outs[e].cpu().numpy(), a[e],
@@ -128,11 +132,20 @@ def test_results_same_alone_and_in_mix(store, graph):
     srv.close()
 
 
-def test_learned_search_served_alone_and_in_mix(store):
+@pytest.mark.parametrize("per_pair", [True, False])
+def test_learned_search_served_alone_and_in_mix(store, per_pair):
     """The learned-start path behind the same loop: every request renders the
-    bytes the learned path renders alone, with pair advances in between."""
-    srv = S.Server(hbm_budget=1 << 34, max_batch=4, n_streams=2, store=store, search="learned")
+    bytes the learned path renders alone, with pair advances in between,
+    with the network once per keyframe pair of a session eye (its output
+    counted in admission) or on every request."""
+    srv = S.Server(hbm_budget=1 << 34, max_batch=4, n_streams=2, store=store, search="learned",
+                   learned_per_pair=per_pair)
     sids = [srv.open_session(SMALL[i % 3], pair=i % 2) for i in range(5)]
+    extra = sum(S.learned_out_footprint(srv.sessions[s].cls.width, srv.sessions[s].cls.height,
+                                        srv.sessions[s].cls.eyes) for s in sids)
+    base = sum(S.session_footprint(srv.sessions[s].cls.width, srv.sessions[s].cls.height,
+                                   srv.sessions[s].cls.eyes) for s in sids)
+    assert sum(srv.sessions[s].footprint for s in sids) == base + (extra if per_pair else 0)
     rng = np.random.default_rng(1)
     log = run_interleaved(srv, sids, 5, rng, {sid: i % 2 for i, sid in enumerate(sids)})
     assert srv.stats.batches >= 2 and sum(srv.sessions[s].advances for s in sids) >= 2
