@@ -346,6 +346,68 @@ def learned_bench(n_frames: int, seed: int, iters: int) -> dict:
     }
 
 
+def learned_headline(rend, ris, new_pair, out, cfg, plain_mpix_s: float, steps: int) -> dict:
+    """Learned starts (main camera; network once per keyframe pair) on the
+    headline workload's renderer and frames: Mpix/s against the plain
+    search's `plain_mpix_s`, and the output against the plain output and the
+    generator's ground truth over the pair's frames."""
+    import torch
+
+    from imagebasedspatiotemporalinterpolation_amd import learned as Ln
+    from imagebasedspatiotemporalinterpolation_amd import scene
+
+    if not Ln.WEIGHTS.exists():
+        return {"error": f"no trained weights at {Ln.WEIGHTS.name}"}
+    params, _ = Ln.load_params(Ln.WEIGHTS)
+    W, H = cfg.width, cfg.height
+    li = Ln.LearnedInit(W, H, 2, params)
+
+    def step():
+        new_pair()
+        for k, ri in enumerate(ris):
+            li.render_pair_frame(rend, out, ri, k == 0)
+
+    for _ in range(3):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    mpix = steps * len(ris) * W * H / (time.perf_counter() - t0) / 1e6
+    # quality over the pair's frames, pixels both methods wrote
+    mse = {"plain_gt": 0.0, "learned_gt": 0.0, "learned_plain": 0.0}
+    n = 0
+    new_pair()
+    for k, ri in enumerate(ris):
+        a = torch.zeros((H, W, 4), dtype=torch.uint8, device="cuda")
+        c = torch.zeros_like(a)
+        rend.render(a, ri)
+        li.render_pair_frame(rend, c, ri, k == 0)
+        torch.cuda.synchronize()
+        a, c = a.cpu().numpy(), c.cpu().numpy()
+        gt = scene.ground_truth(cfg, ri.alpha * cfg.keyframe_spacing)
+        both = (a[..., 3] > 0) & (c[..., 3] > 0)
+        pa, pc, g = (x[..., :3][both].astype(np.float64) / 255.0 for x in (a, c, gt))
+        mse["plain_gt"] += float(((pa - g) ** 2).sum())
+        mse["learned_gt"] += float(((pc - g) ** 2).sum())
+        mse["learned_plain"] += float(((pc - pa) ** 2).sum())
+        n += int(both.sum()) * 3
+    li.close()
+    db = {k: (float("inf") if v == 0 else -10.0 * np.log10(v / max(n, 1))) for k, v in mse.items()}
+    return {
+        "workload": "the headline workload (1920x1080 + 2 aux + soft shadows, fpa 8); learned starts for the "
+                    "main camera, network (480x272) once per keyframe pair",
+        "learned_mpix_per_s": round(mpix, 1),
+        "plain_mpix_per_s": round(plain_mpix_s, 1),
+        "speedup": round(mpix / plain_mpix_s, 3),
+        "psnr_plain_vs_gt_db": round(db["plain_gt"], 3),
+        "psnr_learned_vs_gt_db": round(db["learned_gt"], 3),
+        "delta_psnr_db": round(db["learned_gt"] - db["plain_gt"], 4),
+        "psnr_learned_vs_plain_db": round(db["learned_plain"], 2),
+    }
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -380,11 +442,12 @@ def main():
     out = torch.zeros((H, W, 4), dtype=torch.uint8, device="cuda")
     stream = torch.cuda.current_stream()
     L = N.lib()
-    infos = []
+    infos, ris = [], []
     for b in range(1, fpa):
         vp, ivp, _, _ = scene.camera(cfg, b, 0)
         ri = R.RenderInfo((W, H), b / fpa, vp, ivp, R.AdditionalRenderInfo(fpa, sfpa, False, R.RenderMode(args.mode)),
                           args.iters)
+        ris.append(ri)
         infos.append(ri.to_c())
     h, sp, pitch, optr = rend._h, ctypes.c_void_p(stream.cuda_stream), W * 4, out.data_ptr()
 
@@ -550,6 +613,10 @@ def main():
     if args.learned_frames > 0:
         print("[bench] learned-init vs plain search", file=sys.stderr, flush=True)
         res["learned_init"] = learned_bench(args.learned_frames, args.seed, args.iters)
+        if args.mode in (0, 2, 3):
+            print("[bench] learned starts on the headline workload", file=sys.stderr, flush=True)
+            res["learned_init"]["headline_workload"] = learned_headline(
+                rend, ris, new_pair, out, cfg, value, min(args.steps, 30))
     else:
         res["learned_init"] = None
     if args.stereo_frames > 0:
