@@ -339,7 +339,7 @@ isti_status plan_frame(isti_renderer *r, const isti_render_info *info, uint8_t *
             J.h = d.height;
             J.n_px = d.width * d.height;
             J.weight = sl == 0 ? info->alpha : 1.0f - info->alpha;
-            J.main_slot = sl;
+            J.main_slot = v == 0 ? -1 : sl;  // only aux views' visible-in-main flags are read
             J.inv_w = 1.0 / (double)d.width;
             J.rcp_w = (float)(1.0 / (double)d.width);
             J.write_full = (sc.full_ok && !ext) ? 0 : 1;

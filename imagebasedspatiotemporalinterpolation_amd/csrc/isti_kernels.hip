@@ -70,17 +70,22 @@ __device__ __forceinline__ bool project_pixel(const J_t &J, const M_t &vp_cam, c
     // midpoint below lim (a tie rounds to the even neighbour below). For w
     // finite and non-zero that is |c| <= LIM_MID |w|, and LIM_MID (25 bits)
     // times w (24 bits) is exact in binary64: no division is needed. A zero,
-    // infinite or NaN w takes the IEEE quotients.
-    const float4 rm = mat_vec(vp_main, ws_from);
-    bool visible;
-    if (__builtin_amdgcn_classf(rm.w, CLASS_FINITE_NONZERO)) {
-        const double bound = LIM_MID * __builtin_fabs((double)rm.w);
-        visible = __builtin_fabs((double)rm.x) <= bound && __builtin_fabs((double)rm.y) <= bound &&
-                  __builtin_fabs((double)rm.z) <= bound;
-    } else {
-        const float lim = 1.0f - 1e-3f;
-        const float qx = rm.x / rm.w, qy = rm.y / rm.w, qz = rm.z / rm.w;
-        visible = qx > -lim && qx < lim && qy > -lim && qy < lim && qz > -lim && qz < lim;
+    // infinite or NaN w takes the IEEE quotients. Only an aux view's flag is
+    // ever read (the aux candidate's, renderer.cu:455; the main camera's
+    // result never consults its own), so a main view (main_slot < 0, uniform
+    // per job) skips the test and stores 0.
+    bool visible = false;
+    if (J.main_slot >= 0) {
+        const float4 rm = mat_vec(vp_main, ws_from);
+        if (__builtin_amdgcn_classf(rm.w, CLASS_FINITE_NONZERO)) {
+            const double bound = LIM_MID * __builtin_fabs((double)rm.w);
+            visible = __builtin_fabs((double)rm.x) <= bound && __builtin_fabs((double)rm.y) <= bound &&
+                      __builtin_fabs((double)rm.z) <= bound;
+        } else {
+            const float lim = 1.0f - 1e-3f;
+            const float qx = rm.x / rm.w, qy = rm.y / rm.w, qz = rm.z / rm.w;
+            visible = qx > -lim && qx < lim && qy > -lim && qy < lim && qz > -lim && qz < lim;
+        }
     }
     const float vis = visible ? 1.0f : 0.0f;
 
@@ -142,7 +147,7 @@ __global__ __launch_bounds__(256) void reproject_splat_kernel(const SplatArgs a)
     for (int i = 1; i < a.n_jobs; i++)
         if (b >= a.job[i].block_begin) j = i;
     const SplatJob &J = a.job[j];
-    splat_pixel(J, a.vp_cam, a.vp_main[J.main_slot], a.init_linear != 0, (b - J.block_begin) * SPLAT_BLOCK + threadIdx.x);
+    splat_pixel(J, a.vp_cam, a.vp_main[J.main_slot < 0 ? 0 : J.main_slot], a.init_linear != 0, (b - J.block_begin) * SPLAT_BLOCK + threadIdx.x);
 }
 
 // LDS-binned splat: a block reprojects a 32x8 source tile and first
@@ -170,7 +175,7 @@ __global__ __launch_bounds__(256) void reproject_splat_binned_kernel(const Splat
     if (x < J.w && y < J.h) {
         uint32_t tx, ty;
         unsigned long long key;
-        if (project_pixel(J, a.vp_cam, a.vp_main[J.main_slot], a.init_linear != 0, x, y, y * J.w + x, tx, ty, key)) {
+        if (project_pixel(J, a.vp_cam, a.vp_main[J.main_slot < 0 ? 0 : J.main_slot], a.init_linear != 0, x, y, y * J.w + x, tx, ty, key)) {
             const int bx = (int)tx - (x0 - BIN_M), by = (int)ty - (y0 - BIN_M);
             if (bx >= 0 && bx < BIN_W && by >= 0 && by < BIN_H) atomicMin(&bin[by * BIN_W + bx], key);
             else atomicMin(&J.splat[ty * J.w + tx], key);
@@ -379,7 +384,7 @@ __global__ __launch_bounds__(256) void splat_batch_kernel(const BatchTable *__re
         }
         ISTI_CONST_REF(SplatJob, J, &t->jobs[j]);
         ISTI_CONST_REF(BatchRequest, R, &t->req[J.req]);
-        splat_pixel(J, R.vp_cam, R.vp_main[J.main_slot], R.init_linear != 0, (k - jb) * SPLAT_BLOCK + threadIdx.x);
+        splat_pixel(J, R.vp_cam, R.vp_main[J.main_slot < 0 ? 0 : J.main_slot], R.init_linear != 0, (k - jb) * SPLAT_BLOCK + threadIdx.x);
     }
 }
 

@@ -40,7 +40,7 @@ struct SplatJob {
     unsigned long long *splat;
     uint32_t w, h, n_px, block_begin;
     float weight;
-    int32_t main_slot;  // which main view's VP decides visible_in_main
+    int32_t main_slot;  // which main view's VP decides visible_in_main; < 0: a main view (no flag)
     int32_t req;        // owning request (batched launches)
     float rcp_w;        // 1/w rounded to binary32, for div_px
     int32_t write_full; // 0: full_ff already holds this keyframe pair's flow

@@ -135,7 +135,9 @@ typedef struct isti_renderer isti_renderer;
  * packed: cam_ff (source -> target camera) holds 4 floats per pixel: flow x,
  * y, the moved NDC z = RN32(z + flow z) (the reference keeps the flow z and
  * adds the keyframe z where it reads it, renderer.cu:338-341; the sum is
- * stored instead, so a resolve needs no z fetch) and the visible-in-main flag;
+ * stored instead, so a resolve needs no z fetch) and the visible-in-main flag
+ * (computed for aux views only: a main view's flag is never read, renderer.cu:455,
+ * and is 0);
  * full_ff (source -> other keyframe) 3 floats (x, y, z; the reference's
  * fourth component repeats the flag and is never read).
  * Reference: Renderer members, include/renderer.h:64-92. */

@@ -96,6 +96,17 @@ def flow_err_px(a: np.ndarray, b: np.ndarray) -> float:
     return float(max((d[..., 0] * w / 2).max(initial=0.0), (d[..., 1] * h / 2).max(initial=0.0)))
 
 
+def assert_visible_flags(g: dict, s: dict, key) -> None:
+    """cam_ff's visible-in-main flag (channel 3) of view key = (slot, view):
+    the oracle's flag for an aux view; 0 for a main view, whose flag the
+    resolve never reads (only the aux candidate's, renderer.cu:455) and the
+    splat therefore does not compute."""
+    if key[1] == 0:
+        assert np.all(g["cam_ff"][..., 3] == 0.0), f"main view flag {key}"
+    else:
+        np.testing.assert_array_equal(g["cam_ff"][..., 3], s["cam_ff"][..., 3], err_msg=f"flag {key}")
+
+
 def assert_rgb_parity(gpu: np.ndarray, ref: np.ndarray, min_psnr: float = 50.0):
     diff = np.abs(gpu.astype(np.int16) - ref.astype(np.int16))
     assert int(diff.max(initial=0)) <= 1, f"max abs diff {int(diff.max())} at {np.argwhere(diff.max(-1) > 1)[:5]}"

@@ -44,5 +44,5 @@ def test_edge_sizes_and_aux_counts(oracle_lib, w, h, aw, ah, n_aux, b, mode):
             np.testing.assert_array_equal(g["splat"], s["splat"], err_msg=f"splat {key}")
             assert F.flow_err_px(g["cam_ff"], s["cam_ff"]) <= 1e-3, f"cam_ff {key}"
             assert F.flow_err_px(g["full_ff"], s["full_ff"]) <= 1e-3, f"full_ff {key}"
-            np.testing.assert_array_equal(g["cam_ff"][..., 3], s["cam_ff"][..., 3])
+            F.assert_visible_flags(g, s, key)
     F.assert_rgb_parity(got, ref)
