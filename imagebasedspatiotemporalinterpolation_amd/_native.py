@@ -21,6 +21,7 @@ ISTI_ERR_OOM = 2
 ISTI_ERR_DEVICE = 3
 ISTI_ERR_UNSUPPORTED_SHAPE = 4
 ISTI_MAX_AUX = 4
+ISTI_MAX_FRAMES = 64
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "DEVICE", 4: "UNSUPPORTED_SHAPE"}
 
 EXPORTED_SYMBOLS = (
@@ -29,6 +30,7 @@ EXPORTED_SYMBOLS = (
     "isti_keyframe_set",
     "isti_keyframe_swap",
     "isti_render",
+    "isti_render_frames",
     "isti_render_noalloc",
     "isti_scratch",
     "isti_set_stage_timing",
@@ -139,6 +141,9 @@ def lib() -> ctypes.CDLL:
     L.isti_keyframe_swap.restype = c_int32
     L.isti_render.argtypes = [c_void_p, POINTER(RenderInfoC), c_void_p, c_size_t, c_void_p]
     L.isti_render.restype = c_int32
+    L.isti_render_frames.argtypes = [c_void_p, POINTER(RenderInfoC), c_uint32, POINTER(c_void_p), POINTER(c_size_t),
+                                     c_void_p]
+    L.isti_render_frames.restype = c_int32
     L.isti_render_noalloc.argtypes = [c_void_p, POINTER(RenderInfoC), c_void_p, c_size_t, c_void_p]
     L.isti_render_noalloc.restype = c_int32
     L.isti_scratch.argtypes = [c_void_p, c_int32, c_int32, POINTER(ScratchView)]

@@ -83,6 +83,12 @@ struct isti_renderer {
     // frame whose splat stage isti_learned_features ran last (isti_render_learned resolves it)
     isti_render_info prepared{};
     bool prepared_valid = false;
+    // isti_render_frames: a second set of splat and cam_ff buffers (its full_ff
+    // is the first set's), the two streams consecutive frames alternate on,
+    // and the events that fork them from and join them to the caller's stream
+    Scratch scratch2[2][MAX_VIEWS];
+    hipStream_t pipe[2] = {nullptr, nullptr};
+    hipEvent_t pipe_ev[4] = {};  // fork, flow written, join 0, join 1
     int tile_shape = -1;  // forced resolve tile shape, or -1 (tuned / default)
     int splat_mode = -1;  // forced splat variant (0 direct, 1 LDS-binned), or -1 (tuned / direct)
     std::string err;
@@ -116,6 +122,28 @@ isti_status ensure_scratch(isti_renderer *r, Scratch &s, uint32_t w, uint32_t h)
     if (e == hipSuccess) e = hipMalloc(&s.full_ff, n * sizeof(Flow3));
     if (e != hipSuccess) {
         free_scratch(s);
+        return dev_fail(r, e, "hipMalloc(scratch)");
+    }
+    s.w = w;
+    s.h = h;
+    return ISTI_OK;
+}
+
+// The second buffer set of isti_render_frames: splat and cam_ff only.
+void free_scratch2(Scratch &s) {
+    if (s.splat) (void)hipFree(s.splat);
+    if (s.cam_ff) (void)hipFree(s.cam_ff);
+    s = Scratch{};
+}
+
+isti_status ensure_scratch2(isti_renderer *r, Scratch &s, uint32_t w, uint32_t h) {
+    if (s.w == w && s.h == h && s.splat) return ISTI_OK;
+    free_scratch2(s);
+    const size_t n = (size_t)w * h;
+    hipError_t e = hipMalloc(&s.splat, n * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&s.cam_ff, n * sizeof(float4));
+    if (e != hipSuccess) {
+        free_scratch2(s);
         return dev_fail(r, e, "hipMalloc(scratch)");
     }
     s.w = w;
@@ -301,9 +329,10 @@ struct FramePlan {
 
 // ext: per (slot, view) scratch planes carved from caller-provided memory, or
 // null for the renderer's own. Carved planes may hold another call's data, so
-// the flow toward the other keyframe is always recomputed.
+// the flow toward the other keyframe is always recomputed, unless ext_full
+// says that ext's full_ff planes (and full_ok flags) are the renderer's own.
 isti_status plan_frame(isti_renderer *r, const isti_render_info *info, uint8_t *out, size_t out_pitch,
-                       FramePlan &P, const Scratch (*ext)[MAX_VIEWS] = nullptr) {
+                       FramePlan &P, const Scratch (*ext)[MAX_VIEWS] = nullptr, bool ext_full = false) {
     const int nv = 1 + r->n_aux;
     Scratch (*scratch)[MAX_VIEWS] = r->scratch;
     if (ext) {
@@ -342,7 +371,7 @@ isti_status plan_frame(isti_renderer *r, const isti_render_info *info, uint8_t *
             J.main_slot = v == 0 ? -1 : sl;  // only aux views' visible-in-main flags are read
             J.inv_w = 1.0 / (double)d.width;
             J.rcp_w = (float)(1.0 / (double)d.width);
-            J.write_full = (sc.full_ok && !ext) ? 0 : 1;
+            J.write_full = (sc.full_ok && (!ext || ext_full)) ? 0 : 1;
             J.inv_h = 1.0 / (double)d.height;
             J.inv_vp_from = to_mat(d.inv_viewproj);
             J.vp_to = to_mat(other.viewproj);
@@ -830,6 +859,15 @@ isti_status isti_create(const isti_config *config, isti_renderer **out) {
 isti_status isti_destroy(isti_renderer *r) {
     if (!r) return ISTI_ERR_INVALID_ARG;
     (void)hipSetDevice(r->device);
+    for (hipStream_t &ps : r->pipe)
+        if (ps) {
+            (void)hipStreamSynchronize(ps);
+            (void)hipStreamDestroy(ps);
+        }
+    for (hipEvent_t &ev : r->pipe_ev)
+        if (ev) (void)hipEventDestroy(ev);
+    for (auto &slot : r->scratch2)
+        for (auto &s : slot) free_scratch2(s);
     for (auto &slot : r->scratch)
         for (auto &s : slot) free_scratch(s);
     if (r->row_sums) (void)hipFree(r->row_sums);
@@ -962,6 +1000,83 @@ extern "C" {
 isti_status isti_render(isti_renderer *r, const isti_render_info *info, uint8_t *out, size_t out_pitch,
                         void *stream) {
     return isti_internal_render(r, info, out, out_pitch, stream, nullptr, 0);
+}
+
+isti_status isti_render_frames(isti_renderer *r, const isti_render_info *infos, uint32_t n, uint8_t *const *outs,
+                               const size_t *pitches, void *stream) {
+    if (!r) return ISTI_ERR_INVALID_ARG;
+    if (!infos || !outs || !pitches || n == 0 || n > ISTI_MAX_FRAMES)
+        return fail(r, ISTI_ERR_INVALID_ARG, "infos, outs and pitches must be non-null and n in 1..ISTI_MAX_FRAMES");
+    for (uint32_t i = 0; i < n; i++) {
+        if (!outs[i]) return fail(r, ISTI_ERR_INVALID_ARG, "outs[i] must be non-null");
+        if (infos[i].mode == ISTI_MODE_TIMEWARP)
+            return fail(r, ISTI_ERR_INVALID_ARG, "timewarp frames render with isti_render");
+        const isti_status st = validate(r, &infos[i], pitches[i]);
+        if (st != ISTI_OK) return st;
+        for (uint32_t j = 0; j < i; j++)
+            if (outs[j] == outs[i]) return fail(r, ISTI_ERR_INVALID_ARG, "the frames' outputs must be distinct");
+    }
+    if (n == 1) return isti_internal_render(r, &infos[0], outs[0], pitches[0], stream, nullptr, 0);
+    hipError_t e = hipSetDevice(r->device);
+    if (e != hipSuccess) return dev_fail(r, e, "hipSetDevice");
+    for (hipStream_t &ps : r->pipe)  // default priority, like every stream here
+        if (!ps && (e = hipStreamCreateWithFlags(&ps, hipStreamNonBlocking)) != hipSuccess)
+            return dev_fail(r, e, "hipStreamCreateWithFlags");
+    for (hipEvent_t &ev : r->pipe_ev)
+        if (!ev && (e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess)
+            return dev_fail(r, e, "hipEventCreateWithFlags");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if ((e = hipEventRecord(r->pipe_ev[0], s)) != hipSuccess) return dev_fail(r, e, "hipEventRecord");
+    for (hipStream_t ps : r->pipe)
+        if ((e = hipStreamWaitEvent(ps, r->pipe_ev[0], 0)) != hipSuccess) return dev_fail(r, e, "hipStreamWaitEvent");
+    const bool timing = r->timing;
+    r->timing = false;
+    r->prepared_valid = false;
+    isti_status st = ISTI_OK;
+    bool flow_new = false;  // the first frame writes the pair's flow toward the other keyframe
+    for (uint32_t i = 0; i < n && st == ISTI_OK; i++) {
+        const int k = (int)(i & 1u);
+        FramePlan P;
+        if (k == 0) {
+            st = plan_frame(r, &infos[i], outs[i], pitches[i], P);
+        } else {
+            Scratch set2[2][MAX_VIEWS];
+            for (int sl = 0; sl < 2 && st == ISTI_OK; sl++)
+                for (int v = 0; v <= r->n_aux && st == ISTI_OK; v++) {
+                    Scratch &own = r->scratch[sl][v], &b = r->scratch2[sl][v];
+                    st = ensure_scratch2(r, b, own.w, own.h);
+                    set2[sl][v] = b;
+                    set2[sl][v].full_ff = own.full_ff;
+                    set2[sl][v].full_ok = own.full_ok;
+                }
+            if (st == ISTI_OK) st = plan_frame(r, &infos[i], outs[i], pitches[i], P, set2, true);
+        }
+        if (st != ISTI_OK) break;
+        if (i == 0)
+            for (int j = 0; j < P.nj; j++) flow_new = flow_new || P.jobs[j].write_full != 0;
+        const int shape = tile_shape_for(r, P);
+        set_tiles(P.ra, shape);
+        st = splat_stage(r, P, r->pipe[k], splat_binned_for(r, P));
+        if (st != ISTI_OK) break;
+        if (i == 0 && flow_new && (e = hipEventRecord(r->pipe_ev[1], r->pipe[0])) != hipSuccess) {
+            st = dev_fail(r, e, "hipEventRecord");
+            break;
+        }
+        // the other stream's first resolve reads the flow the first frame wrote
+        if (i == 1 && flow_new && (e = hipStreamWaitEvent(r->pipe[1], r->pipe_ev[1], 0)) != hipSuccess) {
+            st = dev_fail(r, e, "hipStreamWaitEvent");
+            break;
+        }
+        if ((e = launch_resolve(P.ra, P.window, P.always_valid, shape, r->pipe[k])) != hipSuccess)
+            st = dev_fail(r, e, "resolve launch");
+    }
+    r->timing = timing;
+    // join (also after an error, so the caller's stream never runs ahead of enqueued work)
+    for (int k = 0; k < 2; k++) {
+        if ((e = hipEventRecord(r->pipe_ev[2 + k], r->pipe[k])) == hipSuccess) e = hipStreamWaitEvent(s, r->pipe_ev[2 + k], 0);
+        if (e != hipSuccess && st == ISTI_OK) st = dev_fail(r, e, "join");
+    }
+    return st;
 }
 
 isti_status isti_scratch(const isti_renderer *r, int32_t slot, int32_t view, isti_scratch_view *out) {

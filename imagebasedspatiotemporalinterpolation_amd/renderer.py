@@ -201,6 +201,31 @@ class Renderer:
         N.check(N.lib().isti_render(self._h, ctypes.byref(ci), render_buffer.data_ptr(), w * 4,
                                     ctypes.c_void_p(s.cuda_stream)), self._h)
 
+    def render_frames(self, render_buffers, render_infos, stream=None) -> None:
+        """The frames of the current keyframe pair (isti_render_frames): the
+        bytes of render() called for each (buffer, info) in order, with
+        consecutive frames overlapped on two internal streams. Buffers must be
+        distinct contiguous uint8 GPU tensors [H, W, 4]."""
+        import torch
+
+        bufs, infos = list(render_buffers), list(render_infos)
+        if len(bufs) != len(infos) or not bufs:
+            raise ValueError("one render buffer per render info, at least one")
+        n = len(bufs)
+        cinfos = (N.RenderInfoC * n)()
+        outs = (ctypes.c_void_p * n)()
+        pitches = (ctypes.c_size_t * n)()
+        for i, (b, info) in enumerate(zip(bufs, infos)):
+            if not (b.is_cuda and b.dtype == torch.uint8 and b.is_contiguous()):
+                raise ValueError("render buffers must be contiguous uint8 GPU tensors [H, W, 4]")
+            if (int(b.shape[1]), int(b.shape[0])) != (int(info.resolution[0]), int(info.resolution[1])):
+                raise ValueError("a render buffer's shape does not match its render_info.resolution")
+            cinfos[i] = info.to_c()
+            outs[i] = b.data_ptr()
+            pitches[i] = int(b.shape[1]) * 4
+        s = stream if stream is not None else torch.cuda.current_stream()
+        N.check(N.lib().isti_render_frames(self._h, cinfos, n, outs, pitches, ctypes.c_void_p(s.cuda_stream)), self._h)
+
     def autotune(self, render_buffer, render_info: RenderInfo, reps: int = 20, stream=None) -> dict:
         """Times the splat variants and every resolve tile shape on this frame
         and keeps the fastest for renders of its size, aux count and window.

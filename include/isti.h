@@ -181,6 +181,24 @@ isti_status isti_keyframe_swap(isti_renderer *r);
 isti_status isti_render(isti_renderer *r, const isti_render_info *info,
                         uint8_t *out_rgba8, size_t out_pitch_bytes, void *stream);
 
+/* The frames of the current keyframe pair, infos[0..n-1], into outs[i]
+ * (row pitch pitches[i]): the same bytes as n isti_render calls in that
+ * order on `stream`, with consecutive frames on two internal streams and
+ * two sets of splat and cam_ff buffers, so one frame's clear and splat run
+ * beside another frame's resolve (the pair's flow toward the other keyframe
+ * is shared: the first frame computes it when the pair is new, and the
+ * other stream's frames wait for it). Ordered after all work enqueued on
+ * `stream` before the call; work enqueued on `stream` after it sees every
+ * frame. Outputs must be distinct images; modes 0-3 (timewarp frames use
+ * isti_render); 1 <= n <= ISTI_MAX_FRAMES. No counterpart in the reference,
+ * whose frame loop renders one client frame per call (src/main.cu:149-191);
+ * it is for callers that know several client cameras of a pair at once.
+ * Stage timing (isti_set_stage_timing) does not record these frames, and
+ * isti_scratch shows the buffers of the frames at even positions. */
+#define ISTI_MAX_FRAMES 64
+isti_status isti_render_frames(isti_renderer *r, const isti_render_info *infos, uint32_t n,
+                               uint8_t *const *outs, const size_t *pitches, void *stream);
+
 /* Size of the static device scratch of isti_render_noalloc, per GPU. */
 #define ISTI_NOALLOC_SCRATCH_BYTES (64u << 20)
 
