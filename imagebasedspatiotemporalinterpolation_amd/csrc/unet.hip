@@ -719,6 +719,7 @@ struct isti_unet {
     uint16_t *gm = nullptr;   // masked bf16 output gradient (largest size)
     const uint16_t *x = nullptr;
     int adam_t = 0;
+    int shape[NL][2] = {};  // forward wave shape (nt, mt) per 3x3 layer from isti_unet_autotune; 0: heuristic
     std::vector<void *> allocs;
 };
 
@@ -847,11 +848,12 @@ void conv3_shape(int cout, int N, int H, int W, int *nt, int *mt) {
     *nt = shapes[most][0], *mt = shapes[most][1];
 }
 
+// shape: (nt, mt) picked by isti_unet_autotune, or nt <= 0 for conv3_shape's.
 hipError_t launch_conv3_fwd(const Src &s, const uint16_t *w, const float *bias, int cin, int cout, int N, int H, int W,
-                            uint16_t *out_bf, float *out_f, int relu, hipStream_t st) {
+                            uint16_t *out_bf, float *out_f, int relu, const int *shape, hipStream_t st) {
     if (cout % 16 != 0 || cin % 16 != 0) return hipErrorInvalidValue;
-    int nt, mt;
-    conv3_shape(cout, N, H, W, &nt, &mt);
+    int nt = shape ? shape[0] : 0, mt = shape ? shape[1] : 0;
+    if (nt <= 0) conv3_shape(cout, N, H, W, &nt, &mt);
     if (cin == 16) return launch_conv3_kc<16, 2>(nt, mt, s, w, bias, cin, cout, N, H, W, out_bf, out_f, relu, st);
     if (cin == 48) return launch_conv3_kc<48, 6>(nt, mt, s, w, bias, cin, cout, N, H, W, out_bf, out_f, relu, st);
     return cin > 32 ? launch_conv3_kc<64, 0>(nt, mt, s, w, bias, cin, cout, N, H, W, out_bf, out_f, relu, st)
@@ -861,10 +863,11 @@ hipError_t launch_conv3_fwd(const Src &s, const uint16_t *w, const float *bias, 
 // The last 3x3 convolution (16 input and output channels) with the 1x1 head
 // in its epilogue: y (bf16, when non-null) and the head's fp32 output.
 hipError_t launch_conv3_head(const Src &s, const uint16_t *w, const float *bias, int cin, int N, int H, int W,
-                             uint16_t *y, float *out, const uint16_t *head_w, const float *head_b, hipStream_t st) {
+                             uint16_t *y, float *out, const uint16_t *head_w, const float *head_b, const int *shape,
+                             hipStream_t st) {
     if (cin != 16) return hipErrorInvalidValue;
-    int nt, mt;
-    conv3_shape(16, N, H, W, &nt, &mt);
+    int nt = shape ? shape[0] : 0, mt = shape ? shape[1] : 0;
+    if (nt <= 0) conv3_shape(16, N, H, W, &nt, &mt);
     if (mt == 4) return launch_conv3_shape<1, 4, 16, 2, true>(s, w, bias, cin, 16, N, H, W, y, out, 1, head_w, head_b, st);
     if (mt == 2) return launch_conv3_shape<1, 2, 16, 2, true>(s, w, bias, cin, 16, N, H, W, y, out, 1, head_w, head_b, st);
     return launch_conv3_shape<1, 1, 16, 2, true>(s, w, bias, cin, 16, N, H, W, y, out, 1, head_w, head_b, st);
@@ -1028,12 +1031,53 @@ isti_status isti_unet_forward(isti_unet *n, const uint16_t *x, float *out, void 
         const int cin = LAYERS[l].ca + LAYERS[l].cb;
         if (l == NL - 2)
             e = launch_conv3_head(s, n->wb[l], n->p + n->b_off[l], cin, n->N, H, W, n->y[l], out, n->wb[NL - 1],
-                                  n->p + n->b_off[NL - 1], st);
+                                  n->p + n->b_off[NL - 1], n->shape[l], st);
         else
             e = launch_conv3_fwd(s, n->wb[l], n->p + n->b_off[l], cin, LAYERS[l].cout, n->N, H, W, n->y[l], nullptr,
-                                 LAYERS[l].relu, st);
+                                 LAYERS[l].relu, n->shape[l], st);
     }
     return e == hipSuccess ? ISTI_OK : fail(e, "forward");
+}
+
+isti_status isti_unet_autotune(isti_unet *n, int32_t reps, void *stream) {
+    if (!n || reps < 1 || reps > 1000) return ISTI_ERR_INVALID_ARG;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    hipError_t e = hipSetDevice(n->device);
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    for (int i = 0; i < 2 && e == hipSuccess; i++) e = hipEventCreate(&ev[i]);
+    const uint16_t *x_keep = n->x;
+    if (!n->x) n->x = n->y[0];  // any tensor of the input's shape: only the timing is kept
+    static const int shapes[][2] = {{1, 4}, {2, 2}, {4, 1}, {1, 2}, {2, 1}, {1, 1}};
+    for (int l = 0; l < NL - 1 && e == hipSuccess; l++) {
+        const int lv = LEVEL[l], H = n->H >> lv, W = n->W >> lv, cout = LAYERS[l].cout;
+        const int cin = LAYERS[l].ca + LAYERS[l].cb;
+        const Src s = layer_src(n, l);
+        float best = 0.0f;
+        int pick = -1;
+        for (int i = 0; i < 6 && e == hipSuccess; i++) {
+            const int *shp = shapes[i];
+            if ((cout / 16) % shp[0] != 0 || (l == NL - 2 && shp[0] != 1)) continue;
+            auto run = [&]() {
+                return l == NL - 2 ? launch_conv3_head(s, n->wb[l], n->p + n->b_off[l], cin, n->N, H, W, n->y[l],
+                                                       n->dy[0], n->wb[NL - 1], n->p + n->b_off[NL - 1], shp, st)
+                                   : launch_conv3_fwd(s, n->wb[l], n->p + n->b_off[l], cin, cout, n->N, H, W, n->y[l],
+                                                      nullptr, LAYERS[l].relu, shp, st);
+            };
+            e = run();  // warm-up
+            if (e == hipSuccess) e = hipEventRecord(ev[0], st);
+            for (int r = 0; r < reps && e == hipSuccess; r++) e = run();
+            if (e == hipSuccess) e = hipEventRecord(ev[1], st);
+            if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+            float ms = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+            if (e == hipSuccess && (pick < 0 || ms < best)) best = ms, pick = i;
+        }
+        if (pick >= 0) n->shape[l][0] = shapes[pick][0], n->shape[l][1] = shapes[pick][1];
+    }
+    n->x = x_keep;
+    for (hipEvent_t &v : ev)
+        if (v) (void)hipEventDestroy(v);
+    return e == hipSuccess ? ISTI_OK : fail(e, "autotune");
 }
 
 isti_status isti_unet_backward(isti_unet *n, const float *dout, void *stream) {

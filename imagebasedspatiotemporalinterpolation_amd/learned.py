@@ -61,6 +61,7 @@ def _lib():
         L.isti_unet_backward.argtypes = [c_void_p, c_void_p, c_void_p]
         L.isti_unet_zero_grads.argtypes = [c_void_p, c_void_p]
         L.isti_unet_adam.argtypes = [c_void_p, c_float, c_float, c_float, c_float, c_void_p]
+        L.isti_unet_autotune.argtypes = [c_void_p, c_int32, c_void_p]
         L.isti_set_search_capture.argtypes = [c_void_p, c_void_p]
         L.isti_learned_features.argtypes = [c_void_p, POINTER(N.RenderInfoC), c_void_p, c_uint32, c_uint32, c_void_p]
         L.isti_learned_loss.argtypes = [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_float, c_void_p,
@@ -69,7 +70,8 @@ def _lib():
                                           c_void_p, c_void_p, c_size_t, c_void_p]
         for f in ("isti_unet_create", "isti_unet_destroy", "isti_unet_get_params", "isti_unet_set_params",
                   "isti_unet_get_grads", "isti_unet_forward", "isti_unet_backward", "isti_unet_zero_grads",
-                  "isti_unet_adam", "isti_set_search_capture", "isti_learned_features", "isti_learned_loss",
+                  "isti_unet_adam", "isti_unet_autotune", "isti_set_search_capture", "isti_learned_features",
+                  "isti_learned_loss",
                   "isti_render_learned"):
             getattr(L, f).restype = c_int32
         _bound = True
@@ -124,6 +126,11 @@ class UNet:
     def forward(self, x, out, stream=None) -> None:
         """x: int16 (bf16 bits) [batch, height, width, 16]; out: float32 same shape."""
         _ok(_lib().isti_unet_forward(self._h, x.data_ptr(), out.data_ptr(), _stream(stream)), "forward")
+
+    def autotune(self, reps: int = 10, stream=None) -> None:
+        """Picks the fastest wave shape of every forward convolution on this
+        size (same bits for every shape); synchronises, clobbers activations."""
+        _ok(_lib().isti_unet_autotune(self._h, reps, _stream(stream)), "autotune")
 
     def backward(self, dout, stream=None) -> None:
         _ok(_lib().isti_unet_backward(self._h, dout.data_ptr(), _stream(stream)), "backward")
@@ -236,6 +243,7 @@ class LearnedInit:
         self.net = UNet(self.net_w, self.net_h, batch=2, device=device)
         if params is not None:
             self.net.set_params(params)
+        self.net.autotune()
         dev = f"cuda:{device}"
         self.x = torch.zeros((2, self.net_h, self.net_w, CH), dtype=torch.int16, device=dev)
         self.out = torch.zeros((2, self.net_h, self.net_w, CH), dtype=torch.float32, device=dev)
@@ -452,6 +460,7 @@ def time_forward(width: int = 480, height: int = 272, reps: int = 100) -> dict:
     net = UNet(width, height, batch=2)
     if params is not None:
         net.set_params(params)
+    net.autotune()
     g = torch.Generator().manual_seed(0)
     x = torch.randn((2, height, width, CH), generator=g).to(torch.bfloat16).view(torch.int16).cuda()
     out = torch.zeros((2, height, width, CH), dtype=torch.float32, device="cuda")

@@ -56,6 +56,13 @@ isti_status isti_unet_get_grads(isti_unet *net, float *host);
  * activations for isti_unet_backward. */
 isti_status isti_unet_forward(isti_unet *net, const uint16_t *x, float *out, void *stream);
 
+/* Times every wave shape of each forward 3x3 convolution on this network's
+ * size (reps launches each, on `stream`, synchronising it) and keeps the
+ * fastest for later forward passes. Every shape computes the same bits; the
+ * activation buffers hold garbage afterwards (run a forward pass before a
+ * backward pass). */
+isti_status isti_unet_autotune(isti_unet *net, int32_t reps, void *stream);
+
 /* Accumulates parameter gradients for dout (device fp32, same shape as out)
  * of the last forward pass. */
 isti_status isti_unet_backward(isti_unet *net, const float *dout, void *stream);
