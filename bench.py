@@ -178,6 +178,9 @@ def stereo_bench(n_frames: int, seed: int) -> dict:
             per_eye.append(R.RenderInfo((cfg.width, cfg.height), b / fpa, vp, ivp,
                                         R.AdditionalRenderInfo(fpa, 0, False, R.RenderMode.OURS)))
         infos.append(per_eye)
+    # the splat variant (direct or LDS-binned) is tuned on the middle frame of
+    # the pair; both eyes share the size, so the batch takes it for both
+    at = eyes[0][0].autotune(eyes[0][2], infos[len(infos) // 2][0], reps=10)
     batch = R.Batch(max_requests=2, graph=True)
     stream = torch.cuda.current_stream()
 
@@ -210,6 +213,8 @@ def stereo_bench(n_frames: int, seed: int) -> dict:
         "p50_frame_ms": round(float(np.percentile(ms, 50)), 4),
         "p99_frame_ms": round(float(np.percentile(ms, 99)), 4),
         "frames": n_frames,
+        "splat": {"variant": "lds_binned" if at["binned"] == 1 else "direct",
+                  "tuned_clear_splat_ms_per_eye": {"direct": round(at["splat_ms"][0], 4), "lds_binned": round(at["splat_ms"][1], 4)}},
     }
 
 

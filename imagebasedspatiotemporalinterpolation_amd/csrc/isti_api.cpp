@@ -460,6 +460,7 @@ isti_status splat_stage(isti_renderer *r, const FramePlan &P, hipStream_t s, boo
         ca.block_begin[j] = clear_blocks;
         clear_blocks += (J.n_px + 511u) / 512u;
         J.block_begin = n_splat_blocks;
+        J.binned = binned ? 1 : 0;
         n_splat_blocks += splat_blocks(J.w, J.h, binned);
     }
     ca.n_bufs = P.nj;
@@ -616,6 +617,9 @@ isti_status isti_render_batch(isti_batch *b, isti_renderer *const *rs, const ist
         if (st == ISTI_OK) st = plan_frame(r, &infos[i], outs[i], pitches[i], b->plans[i]);
         if (st != ISTI_OK) return bfail(b, st, "request " + std::to_string(i) + ": " + r->err);
         var[i] = resolve_variant(b->plans[i].always_valid, r->n_aux);
+        // each request splats with its renderer's variant (forced or tuned for its size)
+        const int32_t binned = splat_binned_for(r, b->plans[i]) ? 1 : 0;
+        for (int k = 0; k < b->plans[i].nj; k++) b->plans[i].jobs[k].binned = binned;
     }
     std::vector<int> order(n);
     for (int i = 0; i < n; i++) order[i] = i;
@@ -670,7 +674,7 @@ isti_status isti_render_batch(isti_batch *b, isti_renderer *const *rs, const ist
             clear_begin[j] = cb;
             cb += (P.jobs[k].n_px + 511u) / 512u;
             splat_begin[j] = sb;
-            sb += (P.jobs[k].n_px + SPLAT_BLOCK - 1) / SPLAT_BLOCK;
+            sb += splat_blocks(P.jobs[k].w, P.jobs[k].h, P.jobs[k].binned != 0);
         }
     }
     clear_begin[nj] = cb;

@@ -150,45 +150,93 @@ __global__ __launch_bounds__(256) void reproject_splat_kernel(const SplatArgs a)
     splat_pixel(J, a.vp_cam, a.vp_main[J.main_slot < 0 ? 0 : J.main_slot], a.init_linear != 0, (b - J.block_begin) * SPLAT_BLOCK + threadIdx.x);
 }
 
-// LDS-binned splat: a block reprojects a 32x8 source tile and first
-// resolves the atomicMin of every target inside a bin of the tile's
-// footprint +-16 pixels in LDS (64-bit ds_min), sending targets outside the
-// bin straight to HBM; then each non-empty bin cell costs one global
-// atomicMin. The minimum is order-independent, so the buffer is the same
-// as the direct splat's. It pays where many sources land on few cells.
-constexpr int BIN_TW = 32, BIN_TH = 8, BIN_M = 16;
+// LDS-binned splat: a block reprojects a 32x8 source tile and resolves the
+// atomicMin of its targets in an LDS bin (64-bit ds_min) before they reach
+// HBM; each non-empty bin cell then costs one global atomicMin, and targets
+// outside the bin go straight to HBM. The minimum is order-independent, so
+// the buffer is the same as the direct splat's.
+// The bin follows the targets, not the sources: a moving camera shifts a
+// whole tile's targets by tens to hundreds of pixels (45 px on the 1080p
+// workload, 200 px on the fast stereo camera), while within a tile they
+// stay within a few pixels of one common shift. The anchor is the shift of
+// the first splatting source pixel of the block (in the waves' order), and
+// the bin covers the source tile moved by it, plus BIN_M cells a side,
+// which holds 95-98 % of the targets on those workloads.
+constexpr int BIN_TW = 32, BIN_TH = 8, BIN_M = 4;
 constexpr int BIN_W = BIN_TW + 2 * BIN_M, BIN_H = BIN_TH + 2 * BIN_M;
+constexpr int BIN_CELLS = BIN_W * BIN_H;
+constexpr int32_t BIN_NO_ANCHOR = INT32_MIN;
+
+struct SplatBin {
+    unsigned long long cell[BIN_CELLS];
+    int32_t anchor[2][256 / 64];  // per wave: bin origin x, y (before the margin)
+};
+
+// Source tile t (row-major over 32x8 tiles) of job J through the bin. Every
+// thread of the block calls it; it returns with the bin free for the next
+// tile only after a block barrier by the caller.
+template <typename J_t, typename M_t>
+__device__ __forceinline__ void splat_tile_binned(const J_t &J, const M_t &vp_cam, const M_t &vp_main,
+                                                  bool init_linear, uint32_t t, SplatBin &sb) {
+    const uint32_t tiles_x = (J.w + BIN_TW - 1) / BIN_TW;
+    const uint32_t tyi = t / tiles_x;
+    const int x0 = (int)((t - tyi * tiles_x) * BIN_TW), y0 = (int)(tyi * BIN_TH);
+    for (int i = (int)threadIdx.x; i < BIN_CELLS; i += 256) sb.cell[i] = SPLAT_EMPTY;
+    const int lx = (int)(threadIdx.x % BIN_TW), ly = (int)(threadIdx.x / BIN_TW);
+    const uint32_t x = (uint32_t)(x0 + lx), y = (uint32_t)(y0 + ly);
+    uint32_t tx = 0, ty = 0;
+    unsigned long long key = 0;
+    bool hit = false;
+    if (x < J.w && y < J.h) hit = project_pixel(J, vp_cam, vp_main, init_linear, x, y, y * J.w + x, tx, ty, key);
+    // the wave's first splatting lane proposes the bin origin: its target
+    // minus its place in the tile
+    const unsigned long long hits = __ballot(hit);
+    const uint32_t wave = threadIdx.x / 64u;
+    if (hits != 0ull) {
+        const int src = __ffsll((long long)hits) - 1;
+        const int ox = __builtin_amdgcn_readlane((int)tx - lx, src);
+        const int oy = __builtin_amdgcn_readlane((int)ty - ly, src);
+        if ((threadIdx.x & 63u) == 0u) {
+            sb.anchor[0][wave] = ox;
+            sb.anchor[1][wave] = oy;
+        }
+    } else if ((threadIdx.x & 63u) == 0u) {
+        sb.anchor[0][wave] = BIN_NO_ANCHOR;
+    }
+    __syncthreads();
+    int bx0 = BIN_NO_ANCHOR, by0 = 0;
+#pragma unroll
+    for (int w = 256 / 64 - 1; w >= 0; w--) {  // the first wave with a splatting lane
+        if (sb.anchor[0][w] != BIN_NO_ANCHOR) {
+            bx0 = sb.anchor[0][w] - BIN_M;
+            by0 = sb.anchor[1][w] - BIN_M;
+        }
+    }
+    if (bx0 == BIN_NO_ANCHOR) return;  // nothing in this tile splats (block-uniform)
+    if (hit) {
+        const uint32_t bx = tx - (uint32_t)bx0, by = ty - (uint32_t)by0;  // wraps when left of / above the bin
+        if (bx < (uint32_t)BIN_W && by < (uint32_t)BIN_H) atomicMin(&sb.cell[by * BIN_W + bx], key);
+        else atomicMin(&J.splat[ty * J.w + tx], key);
+    }
+    __syncthreads();
+    for (int i = (int)threadIdx.x; i < BIN_CELLS; i += 256) {
+        const unsigned long long k = sb.cell[i];
+        if (k != SPLAT_EMPTY) {  // only in-view targets were binned
+            const int gx = bx0 + i % BIN_W, gy = by0 + i / BIN_W;
+            atomicMin(&J.splat[(uint32_t)gy * J.w + (uint32_t)gx], k);
+        }
+    }
+}
 
 __global__ __launch_bounds__(256) void reproject_splat_binned_kernel(const SplatArgs a) {
-    __shared__ unsigned long long bin[BIN_H * BIN_W];
+    __shared__ SplatBin sb;
     const uint32_t b = xcd_remap(blockIdx.x, a.n_blocks);
     int j = 0;
     for (int i = 1; i < a.n_jobs; i++)
         if (b >= a.job[i].block_begin) j = i;
     const SplatJob &J = a.job[j];
-    const uint32_t tiles_x = (J.w + BIN_TW - 1) / BIN_TW;
-    const uint32_t t = b - J.block_begin, tyi = t / tiles_x;
-    const int x0 = (int)((t - tyi * tiles_x) * BIN_TW), y0 = (int)(tyi * BIN_TH);
-    for (int i = (int)threadIdx.x; i < BIN_H * BIN_W; i += 256) bin[i] = SPLAT_EMPTY;
-    __syncthreads();
-    const uint32_t x = (uint32_t)x0 + (threadIdx.x % BIN_TW), y = (uint32_t)y0 + (threadIdx.x / BIN_TW);
-    if (x < J.w && y < J.h) {
-        uint32_t tx, ty;
-        unsigned long long key;
-        if (project_pixel(J, a.vp_cam, a.vp_main[J.main_slot < 0 ? 0 : J.main_slot], a.init_linear != 0, x, y, y * J.w + x, tx, ty, key)) {
-            const int bx = (int)tx - (x0 - BIN_M), by = (int)ty - (y0 - BIN_M);
-            if (bx >= 0 && bx < BIN_W && by >= 0 && by < BIN_H) atomicMin(&bin[by * BIN_W + bx], key);
-            else atomicMin(&J.splat[ty * J.w + tx], key);
-        }
-    }
-    __syncthreads();
-    for (int i = (int)threadIdx.x; i < BIN_H * BIN_W; i += 256) {
-        const unsigned long long k = bin[i];
-        if (k != SPLAT_EMPTY) {  // only in-view targets were binned
-            const int gx = x0 - BIN_M + i % BIN_W, gy = y0 - BIN_M + i / BIN_W;
-            atomicMin(&J.splat[(uint32_t)gy * J.w + (uint32_t)gx], k);
-        }
-    }
+    splat_tile_binned(J, a.vp_cam, a.vp_main[J.main_slot < 0 ? 0 : J.main_slot], a.init_linear != 0,
+                      b - J.block_begin, sb);
 }
 
 // Learned search starts: the same resolve, with the main camera's searches
@@ -370,6 +418,7 @@ __global__ __launch_bounds__(256) void clear_batch_kernel(const BatchTable *__re
 }
 
 __global__ __launch_bounds__(256) void splat_batch_kernel(const BatchTable *__restrict__ tg) {
+    __shared__ SplatBin sb;
     ISTI_CONST_TABLE(t, tg);
     const XcdRange r = xcd_range(t->splat_blocks);
     if (r.first >= r.end) return;
@@ -384,7 +433,14 @@ __global__ __launch_bounds__(256) void splat_batch_kernel(const BatchTable *__re
         }
         ISTI_CONST_REF(SplatJob, J, &t->jobs[j]);
         ISTI_CONST_REF(BatchRequest, R, &t->req[J.req]);
-        splat_pixel(J, R.vp_cam, R.vp_main[J.main_slot < 0 ? 0 : J.main_slot], R.init_linear != 0, (k - jb) * SPLAT_BLOCK + threadIdx.x);
+        if (J.binned) {  // uniform per job: this block's item is a 32x8 tile
+            splat_tile_binned(J, R.vp_cam, R.vp_main[J.main_slot < 0 ? 0 : J.main_slot], R.init_linear != 0, k - jb,
+                              sb);
+            __syncthreads();  // the next item re-initialises the bin
+        } else {
+            splat_pixel(J, R.vp_cam, R.vp_main[J.main_slot < 0 ? 0 : J.main_slot], R.init_linear != 0,
+                        (k - jb) * SPLAT_BLOCK + threadIdx.x);
+        }
     }
 }
 

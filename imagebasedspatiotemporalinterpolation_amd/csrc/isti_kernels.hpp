@@ -44,6 +44,7 @@ struct SplatJob {
     int32_t req;        // owning request (batched launches)
     float rcp_w;        // 1/w rounded to binary32, for div_px
     int32_t write_full; // 0: full_ff already holds this keyframe pair's flow
+    int32_t binned;     // 1: 32x8 source tiles through an LDS bin (splat_tile_binned), 0: direct
     double inv_w, inv_h;  // RN64(1/w), RN64(1/h) for pixel2CS
     Mat4 inv_vp_from;
     Mat4 vp_to;

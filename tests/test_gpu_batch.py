@@ -105,3 +105,34 @@ def test_batch_rejects_duplicates_and_timewarp(sessions):
     with pytest.raises(IstiError):
         batch.render([(s["r"], _info(s, 1, mode=4), buf)])
     batch.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_mixes_binned_and_direct_splats(sessions, graph):
+    """Requests whose renderers splat through the LDS bin and requests that
+    splat directly share one persistent launch; every output and every splat
+    buffer equals the direct render alone."""
+    batch = R.Batch(max_requests=8, graph=graph)
+    try:
+        for rep in range(2):
+            reqs = _requests(sessions, lambda i: 1 + (i + rep) % 3, lambda i: [0, 0, 1, 0, 3][i], 500 + 10 * rep)
+            expect, splats = [], []
+            for s, info, prev in reqs:
+                s["r"].set_splat_binning(0)
+                expect.append(_alone(s, info, prev))
+                splats.append([s["r"].scratch(sl, v)["splat"] for sl in (0, 1) for v in range(1 + s["r"].n_aux)])
+            for i, (s, _, _) in enumerate(reqs):
+                s["r"].set_splat_binning((i + rep) % 2)
+            bufs = [torch.from_numpy(prev.copy()).cuda() for _, _, prev in reqs]
+            print(f"mixed splat variants rep {rep} graph={graph}", flush=True)
+            batch.render([(s["r"], info, buf) for (s, info, _), buf in zip(reqs, bufs)])
+            torch.cuda.synchronize()
+            for i, ((s, _, _), buf, exp) in enumerate(zip(reqs, bufs, expect)):
+                np.testing.assert_array_equal(buf.cpu().numpy(), exp, err_msg=f"request {i} rep {rep}")
+                got = [s["r"].scratch(sl, v)["splat"] for sl in (0, 1) for v in range(1 + s["r"].n_aux)]
+                for a, b in zip(got, splats[i]):
+                    assert np.array_equal(a, b), f"splat buffer of request {i} rep {rep}"
+    finally:
+        for s in sessions:
+            s["r"].set_splat_binning(-1)
+        batch.close()
