@@ -55,7 +55,8 @@ def parse():
     p.add_argument("--serving-seconds", type=float, default=2.0,
                    help="duration of the 48-session serving mix (0 disables it)")
     p.add_argument("--stereo-frames", type=int, default=60, help="stereo 2x2064x2208 frames timed (0 disables)")
-    p.add_argument("--no-autotune", action="store_true", help="keep the default 16x16 resolve tile")
+    p.add_argument("--no-autotune", action="store_true",
+                   help="keep the default 16x16 resolve tile and the direct splat (also for the stereo section)")
     p.add_argument("--learned-frames", type=int, default=201,
                    help="1080p frames timed for learned-init vs plain (0 disables)")
     p.add_argument("--copy-probe", action="store_true",
@@ -153,7 +154,7 @@ def load_counters(path: Path) -> dict:
         return {}
 
 
-def stereo_bench(n_frames: int, seed: int) -> dict:
+def stereo_bench(n_frames: int, seed: int, autotune: bool = True) -> dict:
     """Stereo HMD workload: both 2064x2208 eyes of a frame in one batched launch per stage."""
     import torch
 
@@ -180,7 +181,7 @@ def stereo_bench(n_frames: int, seed: int) -> dict:
         infos.append(per_eye)
     # the splat variant (direct or LDS-binned) is tuned on the middle frame of
     # the pair; both eyes share the size, so the batch takes it for both
-    at = eyes[0][0].autotune(eyes[0][2], infos[len(infos) // 2][0], reps=10)
+    at = eyes[0][0].autotune(eyes[0][2], infos[len(infos) // 2][0], reps=10) if autotune else None
     batch = R.Batch(max_requests=2, graph=True)
     stream = torch.cuda.current_stream()
 
@@ -213,8 +214,10 @@ def stereo_bench(n_frames: int, seed: int) -> dict:
         "p50_frame_ms": round(float(np.percentile(ms, 50)), 4),
         "p99_frame_ms": round(float(np.percentile(ms, 99)), 4),
         "frames": n_frames,
-        "splat": {"variant": "lds_binned" if at["binned"] == 1 else "direct",
-                  "tuned_clear_splat_ms_per_eye": {"direct": round(at["splat_ms"][0], 4), "lds_binned": round(at["splat_ms"][1], 4)}},
+        "splat": ({"variant": "lds_binned" if at["binned"] == 1 else "direct",
+                   "tuned_clear_splat_ms_per_eye": {"direct": round(at["splat_ms"][0], 4),
+                                                    "lds_binned": round(at["splat_ms"][1], 4)}}
+                  if at else {"variant": "direct", "tuned_clear_splat_ms_per_eye": None}),
     }
 
 
@@ -654,7 +657,7 @@ def main():
         res["learned_init"] = None
     if args.stereo_frames > 0:
         print("[bench] stereo workload", file=sys.stderr, flush=True)
-        res["stereo"] = stereo_bench(args.stereo_frames, args.seed)
+        res["stereo"] = stereo_bench(args.stereo_frames, args.seed, autotune=not args.no_autotune)
     if args.serving_seconds > 0:
         from imagebasedspatiotemporalinterpolation_amd import serving
 

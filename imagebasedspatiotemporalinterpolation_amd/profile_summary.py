@@ -9,6 +9,9 @@ the runs under DIR (made with `--output-format csv`):
   cfetch/cf_counter_collection.csv       --pmc FETCH_SIZE of bench.py --copy-probe
   cwrite/cw_counter_collection.csv       --pmc WRITE_SIZE of bench.py --copy-probe
   sq/sq_counter_collection.csv           --pmc SQ_WAVES SQ_INSTS_VALU ... of bench.py
+  atom/atom_counter_collection.csv       --pmc TCC_ATOMIC_sum TCC_EA0_ATOMIC_sum of bench.py
+                                         (optional: L2 atomic requests, and those sent on
+                                         to memory)
 
 and writes the per-kernel counter file bench.py reads for its roofline
 (--out, default profiles/counters_1080p.json): per launch, the HBM traffic
@@ -28,7 +31,7 @@ from pathlib import Path
 
 # the kernels of the per-frame path whose counters the bench looks up
 KERNELS = re.compile(r"(clear_splat_kernel|reproject_splat_kernel|reproject_splat_binned_kernel|"
-                     r"resolve_kernel<[^>]*>)")
+                     r"resolve_kernel<[^>]*>|clear_batch_kernel|splat_batch_kernel|resolve_batch_kernel<[^>]*>)")
 COPY_BYTES = 64 * 1024 * 1024 * 4  # bench.py --copy-probe: one 256 MiB copy, repeated
 SQ = ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_VMEM_RD", "SQ_INSTS_VMEM_WR", "SQ_INSTS_LDS")
 
@@ -85,9 +88,14 @@ def isolated(trace_csv: Path) -> dict:
     return {k: (len(v), sum(v) / len(v)) for k, v in out.items()}
 
 
-def summarise(root: Path) -> dict:
-    fetch_scale = round(copy_scale(root / "cfetch/cf_counter_collection.csv", "FETCH_SIZE"), 4)
-    write_scale = round(copy_scale(root / "cwrite/cw_counter_collection.csv", "WRITE_SIZE"), 4)
+def summarise(root: Path, scales: tuple | None = None, workload: str | None = None) -> dict:
+    """Per-kernel summary of the runs under root. scales = (fetch, write)
+    byte scales taken from an earlier calibration instead of the copy runs."""
+    if scales is None:
+        fetch_scale = round(copy_scale(root / "cfetch/cf_counter_collection.csv", "FETCH_SIZE"), 4)
+        write_scale = round(copy_scale(root / "cwrite/cw_counter_collection.csv", "WRITE_SIZE"), 4)
+    else:
+        fetch_scale, write_scale = scales
     ks: dict = {}
 
     def entry(full):
@@ -115,6 +123,16 @@ def summarise(root: Path) -> dict:
             v = cs.get(c)
             if v:
                 e[c[3:].lower() if c != "SQ_WAVES" else "waves"] = int(round(sum(v) / len(v)))
+    atom = root / "atom/atom_counter_collection.csv"
+    if atom.exists():
+        for name, cs in counters(atom).items():
+            e = entry(name)
+            if e is None:
+                continue
+            for c, key in (("TCC_ATOMIC_sum", "l2_atomics"), ("TCC_EA0_ATOMIC_sum", "memory_atomics")):
+                v = cs.get(c)
+                if v:
+                    e[key] = int(round(sum(v) / len(v)))
     for e in ks.values():
         if "fetch_kib" in e and "write_kib" in e:
             e["traffic"] = int(round((e["fetch_kib"] * fetch_scale + e["write_kib"] * write_scale) * 1024))
@@ -136,8 +154,8 @@ def summarise(root: Path) -> dict:
             if a in e:
                 e[b] = e.pop(a)
     return {
-        "workload": "bench.py 1080p workload: 1920x1080 + 2 aux 960x540, soft shadows, 7 frames per keyframe pair, "
-                    "search <= 32",
+        "workload": workload or "bench.py 1080p workload: 1920x1080 + 2 aux 960x540, soft shadows, 7 frames per "
+                                "keyframe pair, search <= 32",
         "units": "traffic: HBM bytes per launch = FETCH_SIZE*fetch_scale + WRITE_SIZE*write_scale (KiB -> B); "
                  "instruction counts per launch (all waves)",
         "fetch_scale": fetch_scale,
@@ -150,6 +168,7 @@ def tables(d: dict, simds: int = 1024, clock_hz: float = 2.4e9) -> str:
     rows = ["| Kernel | Launches | Trace avg us (alone) | Traffic MB | SQ_WAVES | VALU / wave | SALU / wave "
             "| VMEM rd / wave | LDS / wave | VALU issue us | VALU issue / duration |",
             "|---|---|---|---|---|---|---|---|---|---|---|"]
+    atoms = [(k, e) for k, e in d["kernels"].items() if "l2_atomics" in e]
     for k, e in d["kernels"].items():
         if "waves" not in e:  # traced only (no counter run launched it)
             continue
@@ -160,6 +179,10 @@ def tables(d: dict, simds: int = 1024, clock_hz: float = 2.4e9) -> str:
                     f"| {e.get('valu_insts', 0) / w:.0f} | {e.get('salu_insts', 0) / w:.0f} "
                     f"| {e.get('vmem_rd_insts', 0) / w:.1f} | {e.get('lds_insts', 0) / w:.1f} | {issue:.1f} "
                     f"| {issue / dur if dur else 0:.2f} |")
+    if atoms:
+        rows += ["", "| Kernel | L2 atomic requests (TCC_ATOMIC_sum) | sent on to memory (TCC_EA0_ATOMIC_sum) |",
+                 "|---|---|---|"]
+        rows += [f"| `{k}` | {e['l2_atomics']} | {e.get('memory_atomics', 0)} |" for k, e in atoms]
     return "\n".join(rows)
 
 
@@ -167,8 +190,15 @@ def _main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("root", help="directory holding the rocprofv3 runs")
     ap.add_argument("--out", default=str(Path(__file__).resolve().parent.parent / "profiles" / "counters_1080p.json"))
+    ap.add_argument("--scales-from", help="take the FETCH/WRITE byte scales from this counter file "
+                                          "instead of copy runs under root")
+    ap.add_argument("--workload", help="workload description stored in the output")
     a = ap.parse_args(argv)
-    d = summarise(Path(a.root))
+    scales = None
+    if a.scales_from:
+        ref = json.loads(Path(a.scales_from).read_text())
+        scales = (ref["fetch_scale"], ref["write_scale"])
+    d = summarise(Path(a.root), scales, a.workload)
     Path(a.out).write_text(json.dumps(d, indent=1) + "\n")
     print(f"fetch_scale {d['fetch_scale']} write_scale {d['write_scale']}")
     print(tables(d))
