@@ -29,6 +29,13 @@ constexpr float EPS_1_LOW_RES = EPS_1_FULL_RES * 4.0f;
 constexpr float EPS_2_FULL_RES = 0.001f;
 constexpr float EPS_2_LOW_RES = 0.004f;
 constexpr float EPS_3 = 0.01f;
+// The smallest binary32 a with RN32(sqrt(a)) >= eps, for each threshold eps
+// above: RN32 o sqrt is non-decreasing, so RN32(sqrt(a)) < eps exactly when
+// a < that value, and an error or distance compares with eps without its
+// square root (the values are checked against IEEE sqrtf on the CPU).
+constexpr float EPS_1_FULL_RES_SQ = 0x1.0c6f7ap-18f;  // 0x368637bd
+constexpr float EPS_1_LOW_RES_SQ = 0x1.0c6f7ap-14f;   // 0x388637bd
+constexpr float EPS_3_SQ = 0x1.a36e2cp-14f;           // 0x38d1b716
 constexpr unsigned long long SPLAT_EMPTY = 0xfefefefefefefefeull;  // memset(0xfe), renderer.cu:22
 constexpr uint32_t SOFTSHADOW_BITS = 2;
 constexpr uint32_t SOFTSHADOW_VAL_MASK = (1u << SOFTSHADOW_BITS) - 1u;

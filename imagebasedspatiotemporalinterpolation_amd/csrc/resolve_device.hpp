@@ -132,13 +132,40 @@ __device__ __forceinline__ void chain_step(Chain &c, int i, int n_iter, float2 p
     if (adv) c.m = at_xy(v.cam_ff, v.w, (uint32_t)t.x, (uint32_t)t.y);
 }
 
-// computeReprojError, renderer.cu:204-215
-__device__ __forceinline__ float reproj_error(uint2 q, const ResolveView &v, float2 pcs_cam, float4 m) {
+// computeReprojError, renderer.cu:204-215, without its final square root:
+// the binary32 sum ex*ex + ey*ey the reference takes sqrtf of. The errors
+// are only ever compared, with each other (sqrt_less) and with EPS_1
+// (against sqrt_threshold squares), which the squared values decide exactly.
+__device__ __forceinline__ float reproj_error2(uint2 q, const ResolveView &v, float2 pcs_cam, float4 m) {
     float ex = cs_coord(q.x, v.inv_w) + m.x - pcs_cam.x;
     float ey = cs_coord(q.y, v.inv_h) + m.y - pcs_cam.y;
     ex *= v.rs_x;
     ey *= v.rs_y;
-    return sqrtf(ex * ex + ey * ey);
+    return ex * ex + ey * ey;
+}
+// The error of a search that did not run: the reference's FLT_MAX. It is
+// only ever compared with the error of a valid candidate (below EPS_1), and
+// +inf orders against those exactly as FLT_MAX does.
+constexpr float ERR2_NONE = __builtin_inff();
+
+// RN32(sqrt(a)) < RN32(sqrt(b)) for squared errors a, b (non-negative, +inf
+// or NaN). RN32 o sqrt is non-decreasing, so a >= b (or a NaN) decides
+// "false". For a >= 2^-100 and b > RN32(a (1 + 2^-18)) the square roots
+// differ by a factor above 1 + 2^-20, far more than the 2^-23 relative width
+// of one binary32 rounding interval, so they round apart: "true". Only
+// near-ties (and tiny a) take the square roots.
+__device__ __forceinline__ bool sqrt_less(float a, float b) {
+    const bool lt = a < b;
+    bool r = lt && a >= 0x1p-100f && b > a * 0x1.00004p0f;
+    if (lt && !r) {
+        // The empty volatile asm keeps the square roots inside this rarely
+        // taken branch (the compiler would otherwise compute them up front
+        // and select).
+        float ra = a, rb = b;
+        asm volatile("" : "+v"(ra), "+v"(rb));
+        r = sqrtf(ra) < sqrtf(rb);
+    }
+    return r;
 }
 
 // The texels colorFromShadow reads at one pixel of a view: colour, and, when
@@ -279,8 +306,8 @@ __device__ __forceinline__ uchar4 second_chance(float weight, const FrameConst &
         const Flow3 m_tf = at(to.full_ff, to.w, ux, uy);
         const Texels tp = load_texels(to, ux, uy, sh_to);
         const float dx = m_ft.x - (-m_tf.x), dy = m_ft.y - (-m_tf.y), dz = m_ft.z - (-m_tf.z);
-        const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
-        if (dist < EPS_3 && fabsf(zt - proj_z) < eps2)
+        const float dist2 = dx * dx + dy * dy + dz * dz;  // sqrtf(dist2) < EPS_3 exactly when dist2 < EPS_3_SQ
+        if (dist2 < EPS_3_SQ && fabsf(zt - proj_z) < eps2)
             out = lerp_to_u8(color_from_shadow(fc.sidx, fc.st, fc.sfpa, fc.no_shadows, to, tp), out, weight);
     }
     return out;
@@ -294,24 +321,31 @@ struct Cand {
     bool valid;
     bool visible_in_main;
     int priority;
-    float error;
+    float error2;  // squared reprojection error (reproj_error2)
     float depth;
     int dir;  // 0: fwd keyframe, 1: bwd keyframe, 2: 0.5 blend of both (ALWAYS_VALID)
 };
 
-// Forward/backward decision of one camera, renderer.cu:366-396.
+// Forward/backward decision of one camera, renderer.cu:366-396, on squared
+// errors fe, be: e < EPS_1 is fe < eps1_sq (eps1_sq = sqrt_threshold(EPS_1)).
+// With exactly one error below EPS_1 the reference's fe < be is decided by
+// which one it is (fe < eps1 <= be or the reverse; a NaN error compares
+// false either way, as in the reference), so the square roots of sqrt_less
+// are only looked at when both are below.
 template <bool ALWAYS_VALID>
 __device__ __forceinline__ Cand decide(float fe, float be, float fd, float bd, bool fv, bool bv, int fprio, int bprio,
-                                       float eps1, float eps2) {
+                                       float eps1_sq, float eps2) {
     Cand k{false, false, 0, 0.f, 0.f, 0};
-    if (fe < eps1 || be < eps1) {
-        const bool both = fe < eps1 && be < eps1;
+    const bool fok = fe < eps1_sq, bok = be < eps1_sq;
+    if (fok || bok) {
+        const bool both = fok && bok;
         const bool similar = fabsf(depth_to_z(fd) - depth_to_z(bd)) < eps2;
-        const bool choose_fwd = (both && !similar && fd < bd) || ((!both || similar) && fe < be);
+        const bool f_less = both ? sqrt_less(fe, be) : (fok && !__builtin_isnan(be));
+        const bool choose_fwd = (both && !similar && fd < bd) || ((!both || similar) && f_less);
         k.valid = true;
         k.visible_in_main = choose_fwd ? (fv || (both && similar && bv)) : (bv || (both && similar && fv));
         k.priority = choose_fwd ? fprio : bprio;
-        k.error = choose_fwd ? fe : be;
+        k.error2 = choose_fwd ? fe : be;
         k.depth = choose_fwd ? fd : bd;
         k.dir = choose_fwd ? 0 : 1;
     } else if (ALWAYS_VALID) {
@@ -539,16 +573,16 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
 #pragma unroll
         for (int d = 0; d < 2; d++) {
             const ResolveView &v = d ? a.bwd[c] : a.fwd[c];
-            err[c][d] = FLT_MAX;
+            err[c][d] = ERR2_NONE;
             vis[c][d] = false;
             // the search result, or the window pixel of a search that did not
-            // run (its depth is never read: its error is FLT_MAX)
+            // run (its depth is never read: its error is the reference's FLT_MAX)
             const uint32_t qx = ch[c][d].h0 & 0xFFFFu, qy = ch[c][d].h0 >> 16;
             si[c][d] = SplatInfo{0.0f, qx, qy};
             if (ran[c][d]) {
                 const float4 m = at(v.cam_ff, v.w, qx, qy);
                 si[c][d].depth = z_to_depth(m.z);  // m.z = z + flow z (the splat's sum)
-                err[c][d] = reproj_error(make_uint2(qx, qy), v, cpcs[c], m);
+                err[c][d] = reproj_error2(make_uint2(qx, qy), v, cpcs[c], m);
                 vis[c][d] = m.w > 0.0f;
             }
         }
@@ -563,7 +597,7 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
         for (int d = 0; d < 2; d++) {
             const bool hidden = window0[d].depth > 0.0f &&
                                 !(depth_to_z(si[0][d].depth) - depth_to_z(window0[d].depth) < EPS_2_FULL_RES);
-            redo[d] = learned[d] && !(err[0][d] < EPS_1_FULL_RES && !hidden);
+            redo[d] = learned[d] && !(err[0][d] < EPS_1_FULL_RES_SQ && !hidden);
             if (redo[d]) {
                 si[0][d] = window0[d];
                 ran[0][d] = window0[d].depth > 0.0f && (d == 1 || fc.alpha < 1.0f);
@@ -580,7 +614,7 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
         for (int d = 0; d < 2; d++) {
             if (!redo[d]) continue;
             const ResolveView &v = d ? a.bwd[0] : a.fwd[0];
-            err[0][d] = FLT_MAX;
+            err[0][d] = ERR2_NONE;
             vis[0][d] = false;
             if (ran[0][d]) {
                 const uint32_t qx = ch[0][d].h0 & 0xFFFFu, qy = ch[0][d].h0 >> 16;
@@ -588,7 +622,7 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
                 si[0][d].x = qx;
                 si[0][d].y = qy;
                 si[0][d].depth = z_to_depth(m.z);
-                err[0][d] = reproj_error(make_uint2(qx, qy), v, cpcs[0], m);
+                err[0][d] = reproj_error2(make_uint2(qx, qy), v, cpcs[0], m);
                 vis[0][d] = m.w > 0.0f;
             }
         }
@@ -616,13 +650,13 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
     if (a.q_out) {  // capture of the main camera's search results
 #pragma unroll
         for (int d = 0; d < 2; d++)
-            a.q_out[2 * (y * a.w + x) + d] = err[0][d] < EPS_1_FULL_RES ? pack_px(si[0][d].x, si[0][d].y) : 0xffffffffu;
+            a.q_out[2 * (y * a.w + x) + d] = err[0][d] < EPS_1_FULL_RES_SQ ? pack_px(si[0][d].x, si[0][d].y) : 0xffffffffu;
     }
     // 3.-4. per-camera decision (:366-396), folded at once into the running
     // aux selection (:429-451) so no per-camera array outlives the loop
     // (a dynamically indexed array would live in scratch memory).
     Cand main_k = decide<MAV>(err[0][0], err[0][1], si[0][0].depth, si[0][1].depth, vis[0][0], vis[0][1],
-                              a.fwd[0].priority, a.bwd[0].priority, EPS_1_FULL_RES, EPS_2_FULL_RES);
+                              a.fwd[0].priority, a.bwd[0].priority, EPS_1_FULL_RES_SQ, EPS_2_FULL_RES);
     bool aux_valid = false;
     Cand aux_k = Cand{false, false, 0, 0.f, 0.f, 0};
     SplatInfo aux_f = si[0][0], aux_b = si[0][1];
@@ -630,14 +664,14 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
 #pragma unroll
     for (int c = 1; c < NC; c++) {
         const Cand kc = decide<false>(err[c][0], err[c][1], si[c][0].depth, si[c][1].depth, vis[c][0], vis[c][1],
-                                      a.fwd[c].priority, a.bwd[c].priority, EPS_1_LOW_RES, EPS_2_LOW_RES);
+                                      a.fwd[c].priority, a.bwd[c].priority, EPS_1_LOW_RES_SQ, EPS_2_LOW_RES);
         bool take = false;
         if (!aux_valid) {
             take = kc.valid;  // an invalid first result leaves "no valid best"
         } else if (kc.valid) {
             const bool similar = fabsf(depth_to_z(aux_k.depth) - depth_to_z(kc.depth)) < EPS_2_LOW_RES;
             take = (!similar && kc.depth < aux_k.depth) ||
-                   (similar && kc.priority >= aux_k.priority && kc.error < aux_k.error);
+                   (similar && kc.priority >= aux_k.priority && sqrt_less(kc.error2, aux_k.error2));
         }
         if (take) {
             aux_valid = true;

@@ -192,3 +192,78 @@ def test_visibility_bound_without_division():
                 if name == "near":  # the wrong bound (lim itself) differs here
                     assert not np.array_equal(ref, _visible_fast(cc, ww, float(lim)))
     assert mixed >= 4  # the near-bound sets hold both outcomes
+
+
+# ---------------------------------------------------------------------------
+# Reprojection errors and the second-chance flow distance compared without
+# their square roots (resolve_device.hpp: reproj_error2, sqrt_less, decide,
+# second_chance; renderer.cu:204-215, 315-324, 366-396, 441-447).
+# ---------------------------------------------------------------------------
+EPS_SQ = {  # isti_device.hpp: bit patterns of EPS_1_FULL_RES_SQ, EPS_1_LOW_RES_SQ, EPS_3_SQ
+    0.002: 0x368637BD,
+    0.008: 0x388637BD,
+    0.01: 0x38D1B716,
+}
+
+
+def test_squared_thresholds_are_the_exact_boundaries():
+    """RN32(sqrt(a)) < eps exactly when a < T (RN32 o sqrt is monotone, so the
+    boundary pair decides every a), for the three thresholds of the resolve."""
+    for eps, bits in EPS_SQ.items():
+        e = np.float32(eps)
+        t = np.array([bits], np.uint32).view(np.float32)[0]
+        below = np.nextafter(t, np.float32(0))
+        assert np.sqrt(t) >= e and np.sqrt(below) < e, eps
+        # and around it, every binary32 within 2^16 ulps either side
+        a = (np.arange(-65536, 65537, dtype=np.int64) + bits).astype(np.uint32).view(np.float32)
+        assert np.array_equal(np.sqrt(a) < e, a < t), eps
+    assert np.float32(0.002) * np.float32(4.0) == np.float32(0.008)  # EPS_1_LOW_RES = 4 * EPS_1_FULL_RES
+
+
+def _sqrt_less(a, b):
+    """The kernels' sqrt_less, vectorised: a < b, then the 2^-18 margin, then the square roots."""
+    lt = a < b
+    with np.errstate(over="ignore", invalid="ignore"):
+        clear = (a >= np.float32(2.0 ** -100)) & (b > a * np.float32(1 + 2.0 ** -18))
+        exact = np.sqrt(a) < np.sqrt(b)
+    return lt & (clear | exact)
+
+
+def test_sqrt_less_equals_comparing_square_roots():
+    """sqrt_less(a, b) == (RN32(sqrt(a)) < RN32(sqrt(b))) on near-ties (b a few
+    ulps above a, where the roots often round together), on pairs just above and
+    below the margin, on every magnitude (subnormals included) and on 0, +inf and
+    NaN. Both sides run in binary32 with IEEE square roots."""
+    rng = np.random.default_rng(7)
+    bits = rng.integers(1, 0x7F800000, 2_000_000, dtype=np.int64)
+    a = bits.astype(np.uint32).view(np.float32)
+    pairs = [(a, (bits + rng.integers(-8, 9, bits.size)).clip(0, 0x7F800000).astype(np.uint32).view(np.float32))]
+    for k in (-2, -1, 0, 1, 2):  # around the margin b = RN32(a (1 + 2^-18))
+        m = (a * np.float32(1 + 2.0 ** -18)).view(np.uint32).astype(np.int64) + k
+        pairs.append((a, m.clip(0, 0x7F800000).astype(np.uint32).view(np.float32)))
+    pairs.append((a, rng.permutation(a)))
+    sub = rng.integers(0, 1 << 23, 200_000, dtype=np.int64)
+    sa = sub.astype(np.uint32).view(np.float32)
+    pairs.append((sa, (sub + rng.integers(0, 4, sub.size)).astype(np.uint32).view(np.float32)))
+    special = np.array([0.0, np.inf, np.nan, 1e-45, 4e-6, 6.4e-5, 3.4e38], np.float32)
+    x, y = np.meshgrid(special, special)
+    pairs.append((x.ravel(), y.ravel()))
+    for p, q in pairs:
+        with np.errstate(invalid="ignore"):
+            ref = np.sqrt(p) < np.sqrt(q)
+        assert np.array_equal(_sqrt_less(p, q), ref)
+        with np.errstate(invalid="ignore"):
+            assert np.array_equal(_sqrt_less(q, p), np.sqrt(q) < np.sqrt(p))
+
+
+def test_one_valid_error_decides_the_comparison():
+    """decide(): with exactly one of fe, be below EPS_1 the reference's fe < be is
+    "fe is the valid one and be is not NaN"; the kernel reads it off the flags."""
+    e1 = np.float32(0.002)
+    vals = np.array([0.0, 1e-4, 0.0019, 0.002, 0.0021, 1.0, np.float32(3.4e38), np.inf, np.nan], np.float32)
+    fe, be = np.meshgrid(vals, vals)
+    fe, be = fe.ravel(), be.ravel()
+    with np.errstate(invalid="ignore"):
+        fok, bok = fe < e1, be < e1
+        one = fok ^ bok
+        assert np.array_equal((fe < be)[one], (fok & ~np.isnan(be))[one])
