@@ -249,7 +249,7 @@ __global__ __launch_bounds__(256) void resolve_learned_kernel(const ResolveArgs 
     const ResolveArgs &a = args;
 #endif
     __shared__ ResolveLds<1, NAUX> lds;
-    resolve_tile<1, false, NAUX, true>(a, xcd_remap(blockIdx.x, a.n_tiles), lds);
+    resolve_tile<1, false, NAUX, true>(a, xcd_stripe_tile(blockIdx.x, a.n_tiles, a.tiles_x), lds);
 }
 
 // ---------------------------------------------------------------------------
