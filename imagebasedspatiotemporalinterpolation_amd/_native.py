@@ -50,7 +50,7 @@ EXPORTED_SYMBOLS = (
 )
 ISTI_N_TILE_SHAPES = 4
 ISTI_AUTOTUNE_TIMES = ISTI_N_TILE_SHAPES + 2
-TILE_SHAPES = ((16, 16), (32, 8), (64, 4), (8, 32))
+TILE_SHAPES = ((16, 16), (32, 8), (8, 8), (8, 32))
 
 
 class ViewDesc(ctypes.Structure):

@@ -661,12 +661,13 @@ isti_status isti_render_batch(isti_batch *b, isti_renderer *const *rs, const ist
         BatchRequest &R = req[q];
         std::memset(&R, 0, sizeof(R));
         R.args = P.ra;
+        set_tiles(R.args, BATCH_TILE_SHAPE);  // the batched resolve's tile grid
         R.vp_cam = P.vp_cam;
         R.vp_main[0] = P.vp_main[0];
         R.vp_main[1] = P.vp_main[1];
         R.init_linear = P.init_linear;
         tile_begin[q] = tb;
-        tb += P.ra.n_tiles;
+        tb += R.args.n_tiles;
         mask |= 1u << var[order[q]];
         for (int k = 0; k < P.nj; k++, j++) {
             jobs[j] = P.jobs[k];

@@ -445,8 +445,9 @@ __global__ __launch_bounds__(256) void splat_batch_kernel(const BatchTable *__re
 }
 
 template <int W, bool MAV, int NAUX>
-__global__ __launch_bounds__(256) void resolve_batch_kernel(const BatchTable *__restrict__ tg, int variant) {
-    __shared__ ResolveLds<W, NAUX> lds;
+__global__ __launch_bounds__(BATCH_TILE_W * BATCH_TILE_H) __attribute__((amdgpu_waves_per_eu(resolve_min_waves(W, NAUX))))
+void resolve_batch_kernel(const BatchTable *__restrict__ tg, int variant) {
+    __shared__ ResolveLds<W, NAUX, BATCH_TILE_W, BATCH_TILE_H> lds;
     ISTI_CONST_TABLE(t, tg);
     const uint32_t r0 = t->var_req_begin[variant], r1 = t->var_req_begin[variant + 1];
     const uint32_t base = t->tile_begin[r0];
@@ -462,7 +463,10 @@ __global__ __launch_bounds__(256) void resolve_batch_kernel(const BatchTable *__
 #else
         const ResolveArgs &a = t->req[q].args;
 #endif
-        resolve_tile<W, MAV, NAUX>(a, base + k - tile_begin[q], lds);
+        // a request's tiles in stripe-major order, so an XCD's contiguous
+        // range of items covers whole vertical stripes (xcd_stripe_tile)
+        resolve_tile<W, MAV, NAUX, false, BATCH_TILE_W, BATCH_TILE_H>(
+            a, stripe_major_tile(base + k - tile_begin[q], a.n_tiles, a.tiles_x), lds);
         __syncthreads();  // the next tile restages LDS
     }
 }
@@ -626,13 +630,14 @@ hipError_t launch_resolve_batch(const BatchTable *t, int variant, uint32_t grid,
     void *fn = resolve_batch_ptr(variant);
     if (!fn) return hipErrorInvalidValue;
     void *args[] = {&t, &variant};
-    return hipLaunchKernel(fn, dim3(grid), dim3(TILE_W * TILE_H), args, 0, s);
+    return hipLaunchKernel(fn, dim3(grid), dim3(BATCH_TILE_W * BATCH_TILE_H), args, 0, s);
 }
 
 int resolve_batch_blocks_per_cu(int variant) {
     void *fn = resolve_batch_ptr(variant);
     int n = 0;
-    if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, TILE_W * TILE_H, 0) != hipSuccess) return 1;
+    if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, BATCH_TILE_W * BATCH_TILE_H, 0) != hipSuccess)
+        return 1;
     return n > 0 ? n : 1;
 }
 

@@ -17,12 +17,15 @@ constexpr int MAX_VIEWS = 1 + ISTI_MAX_AUX;        // main + aux per slot
 constexpr int MAX_SPLAT_JOBS = 2 * MAX_VIEWS;       // (slot, view) pairs
 constexpr uint32_t SPLAT_BLOCK = 256;               // source pixels per K1 block
 constexpr uint32_t TILE_W = 16, TILE_H = 16;        // default resolve tile (one block)
-constexpr uint32_t RESOLVE_THREADS = TILE_W * TILE_H;
 // Resolve tile shapes the single-render path is built for (index = shape id;
-// every shape has RESOLVE_THREADS pixels and gives identical results).
+// one thread per pixel; every shape gives identical results).
 constexpr int N_TILE_SHAPES = 4;
-constexpr uint32_t TILE_SHAPE_W[N_TILE_SHAPES] = {16, 32, 64, 8};
-constexpr uint32_t TILE_SHAPE_H[N_TILE_SHAPES] = {16, 8, 4, 32};
+constexpr uint32_t TILE_SHAPE_W[N_TILE_SHAPES] = {16, 32, 8, 8};
+constexpr uint32_t TILE_SHAPE_H[N_TILE_SHAPES] = {16, 8, 8, 32};
+// Tile shape of the persistent batched resolve (16x16; one-wave 8x8 tiles
+// were slower there: 17.6 against 18.8 Gpx/s on the batched stereo frames).
+constexpr int BATCH_TILE_SHAPE = 0;
+constexpr uint32_t BATCH_TILE_W = TILE_SHAPE_W[BATCH_TILE_SHAPE], BATCH_TILE_H = TILE_SHAPE_H[BATCH_TILE_SHAPE];
 
 // Flow toward the other keyframe (full_ff): x, y, z only. Its fourth
 // component (the visibility flag, also in cam_ff.w) is never read, so the

@@ -33,8 +33,8 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
 // than across; per-XCD bands of whole rows measured 10 % slower on the
 // 1080p + 2 aux workload). Stripes hold the contiguous ranges xcd_remap
 // deals, in stripe-major order, so every tile is taken exactly once.
-__device__ __forceinline__ uint32_t xcd_stripe_tile(uint32_t b, uint32_t n_tiles, uint32_t tiles_x) {
-    const uint32_t L = xcd_remap(b, n_tiles);  // position in stripe-major order
+// stripe_major_tile: tile id of position L in that stripe-major order.
+__device__ __forceinline__ uint32_t stripe_major_tile(uint32_t L, uint32_t n_tiles, uint32_t tiles_x) {
     const uint32_t rows = n_tiles / tiles_x;
     uint32_t c0 = 0, c1 = 0, acc = 0;
     for (uint32_t s = 0; s < 8; s++) {
@@ -47,6 +47,17 @@ __device__ __forceinline__ uint32_t xcd_stripe_tile(uint32_t b, uint32_t n_tiles
     const uint32_t l = L - acc, w = c1 - c0;  // w > 0: the stripe holding L is not empty
     return (l / w) * tiles_x + c0 + l % w;
 }
+__device__ __forceinline__ uint32_t xcd_stripe_tile(uint32_t b, uint32_t n_tiles, uint32_t tiles_x) {
+    return stripe_major_tile(xcd_remap(b, n_tiles), n_tiles, tiles_x);
+}
+
+// Occupancy floor of the resolve variants (waves per SIMD). The pass spends
+// most of its time issuing VALU instructions and the rest waiting on
+// dependent gathers, so waves in flight pay: the 2-aux window-1 variant fits
+// 64 VGPRs without spilling and runs at 8 waves, the 3-aux one at 6 (79
+// VGPRs). A floor that forces spills costs more than the waves gain. 1 = no
+// floor.
+constexpr int resolve_min_waves(int w, int naux) { return w != 1 ? 1 : (naux == 2 ? 8 : (naux == 3 ? 6 : 1)); }
 
 // ---------------------------------------------------------------------------
 // K2: resolve (renderer.cu:148-465)
@@ -84,13 +95,13 @@ __device__ __forceinline__ SplatInfo init_search_lds(const unsigned long long *t
 // ceil(2^20 / SPAN) on the 24-bit multiplier: for i < 2^10 the excess
 // i (M - 2^20 / SPAN) / 2^20 < 2^-10 stays below 1 / SPAN (SPAN < 2^10), so
 // the quotient is exact.
-template <int SPAN>
+template <int SPAN, int NTH>
 __device__ __forceinline__ void stage_splat(unsigned long long *dst, const ResolveView &v, int ox, int oy, int rw,
                                             int rh) {
     constexpr uint32_t M = ((1u << 20) + SPAN - 1) / SPAN;
     static_assert(SPAN < 1024, "staged row too long for the 24-bit row split");
     const int n = SPAN * rh;  // rh <= ROWS, and SPAN * ROWS < 2^10 (ResolveLds)
-    for (int i = (int)threadIdx.x; i < n; i += (int)RESOLVE_THREADS) {
+    for (int i = (int)threadIdx.x; i < n; i += NTH) {
         const int yy = (int)(__umul24((uint32_t)i, M) >> 20), xx = i - yy * SPAN;
         const int gx = ox + xx, gy = oy + yy;
         const bool in = gx >= 0 && gx < (int)v.w && gy >= 0 && gy < (int)v.h;
@@ -418,12 +429,13 @@ struct ResolveLds {
     unsigned long long splat[NC][2][SPAN * ROWS];
 };
 
-// One TW x TH tile (TW * TH == RESOLVE_THREADS) of one render. Every thread
-// of the block must call it (the LDS staging ends in a barrier); pixels
-// outside the image return early.
+// One TW x TH tile of one render, one thread per pixel (TW * TH threads: 64,
+// one wave, or up to 256). Every thread of the block must call it (the LDS
+// staging ends in a barrier); pixels outside the image return early.
 template <int W, bool MAV, int NAUX, bool LEARNED = false, int TW = TILE_W, int TH = TILE_H, typename A>
 __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveLds<W, NAUX, TW, TH> &lds) {
-    static_assert(TW * TH == (int)RESOLVE_THREADS && (TW & (TW - 1)) == 0, "tile shape");
+    constexpr int NTH = TW * TH;  // threads of the block
+    static_assert(NTH % 64 == 0 && NTH <= 256 && (TW & (TW - 1)) == 0, "tile shape");
     constexpr int NC = 1 + NAUX;
     constexpr int SPAN = TW + 2 * W;
     auto &s_splat = lds.splat;
@@ -445,8 +457,8 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
         org_y[c] = (int)cy0 - W;
         span_x[c] = (int)(div_small(xl, v.ratio_x, v.magic_x) - cx0) + 1 + 2 * W;
         span_y[c] = (int)(div_small(yl, v.ratio_y, v.magic_y) - cy0) + 1 + 2 * W;
-        stage_splat<SPAN>(s_splat[c][0], a.fwd[c], org_x[c], org_y[c], span_x[c], span_y[c]);
-        stage_splat<SPAN>(s_splat[c][1], a.bwd[c], org_x[c], org_y[c], span_x[c], span_y[c]);
+        stage_splat<SPAN, NTH>(s_splat[c][0], a.fwd[c], org_x[c], org_y[c], span_x[c], span_y[c]);
+        stage_splat<SPAN, NTH>(s_splat[c][1], a.bwd[c], org_x[c], org_y[c], span_x[c], span_y[c]);
     }
     __syncthreads();
     // 1a. aux cameras: a window start depends only on the aux pixel, which
@@ -454,7 +466,7 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
     // tile is initialised once and its start replaces the staged cells (packed
     // like a splat key) for the pixels to read.
     if constexpr (NAUX > 0) {
-        constexpr int PER = (NAUX * 2 * TW * TH + (int)RESOLVE_THREADS - 1) / (int)RESOLVE_THREADS;
+        constexpr int PER = (NAUX * 2 * TW * TH + NTH - 1) / NTH;
         unsigned long long res[PER];
         int nx[NC], cnt[NC], first[NC + 1];
         float rcp_nx[NC];  // RN32(1/nx): cell rows by div_px instead of an integer divide
@@ -468,7 +480,7 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
         }
 #pragma unroll
         for (int r = 0; r < PER; r++) {
-            const int k = (int)threadIdx.x + r * (int)RESOLVE_THREADS;
+            const int k = (int)threadIdx.x + r * NTH;
             res[r] = SPLAT_EMPTY;
 #pragma unroll
             for (int c = 1; c < NC; c++) {
@@ -483,7 +495,7 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < PER; r++) {
-            const int k = (int)threadIdx.x + r * (int)RESOLVE_THREADS;
+            const int k = (int)threadIdx.x + r * NTH;
 #pragma unroll
             for (int c = 1; c < NC; c++) {
                 if (k >= first[c] && k < first[c + 1]) {

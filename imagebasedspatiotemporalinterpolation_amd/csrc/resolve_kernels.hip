@@ -6,16 +6,8 @@
 
 namespace isti {
 
-// Occupancy floor per variant (waves per SIMD). The pass spends most of its
-// time issuing VALU instructions (about three quarters of a launch on the
-// 1080p + 2 aux workload) and the rest waiting on dependent gathers, so
-// waves in flight pay: the 2-aux window-1 variant fits 64 VGPRs without
-// spilling and runs at 8 waves, the 3-aux one at 6 (79 VGPRs). A floor that
-// forces spills costs more than the waves gain. 1 = no floor.
-constexpr int resolve_min_waves(int w, int naux) { return w != 1 ? 1 : (naux == 2 ? 8 : (naux == 3 ? 6 : 1)); }
-
 template <int W, bool MAV, int NAUX, int TW, int TH>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(resolve_min_waves(W, NAUX))))
+__global__ __launch_bounds__(TW * TH) __attribute__((amdgpu_waves_per_eu(resolve_min_waves(W, NAUX))))
 void resolve_kernel(const ResolveArgs args) {
     // Read the argument block in place (constant address space) instead of
     // through a by-value copy, which the compiler may spill to scratch.
@@ -30,7 +22,7 @@ void resolve_kernel(const ResolveArgs args) {
 
 template <int W, bool MAV, int TW, int TH>
 static hipError_t launch_resolve_n(const ResolveArgs &a, hipStream_t s) {
-    const dim3 grid(a.n_tiles), block(RESOLVE_THREADS);
+    const dim3 grid(a.n_tiles), block(TW * TH);
     switch (a.n_aux) {
         case 0: hipLaunchKernelGGL((resolve_kernel<W, MAV, 0, TW, TH>), grid, block, 0, s, a); break;
         case 1: hipLaunchKernelGGL((resolve_kernel<W, MAV, 1, TW, TH>), grid, block, 0, s, a); break;
@@ -46,7 +38,7 @@ static hipError_t launch_resolve_shape(const ResolveArgs &a, int shape, hipStrea
     switch (shape) {
         case 0: return launch_resolve_n<W, MAV, 16, 16>(a, s);
         case 1: return launch_resolve_n<W, MAV, 32, 8>(a, s);
-        case 2: return launch_resolve_n<W, MAV, 64, 4>(a, s);
+        case 2: return launch_resolve_n<W, MAV, 8, 8>(a, s);
         case 3: return launch_resolve_n<W, MAV, 8, 32>(a, s);
         default: return hipErrorInvalidValue;
     }

@@ -219,16 +219,19 @@ isti_status isti_render_frames(isti_renderer *r, const isti_render_info *infos, 
 isti_status isti_render_noalloc(isti_renderer *r, const isti_render_info *info,
                                 uint8_t *out_rgba8, size_t out_pitch_bytes, void *stream);
 
-/* Resolve tile shapes (render pixels per workgroup, 256 each): 0 = 16x16,
- * 1 = 32x8, 2 = 64x4, 3 = 8x32. Every shape gives identical results; they
- * differ in speed (halo staging, gather locality, wave shape). No counterpart
- * in the reference (16x16 blocks, src/renderer.cu:676-693). */
+/* Resolve tile shapes (render pixels per workgroup, one thread each):
+ * 0 = 16x16, 1 = 32x8, 2 = 8x8 (one wave per workgroup), 3 = 8x32. Every
+ * shape gives identical results; they differ in speed (halo staging, gather
+ * locality, wave shape, and how long a workgroup's finished waves wait for
+ * its slowest one). No counterpart in the reference (16x16 blocks,
+ * src/renderer.cu:676-693). */
 #define ISTI_N_TILE_SHAPES 4
 
 /* Splat variants: 0 = direct (one 64-bit atomicMin into HBM per splatting
  * source pixel), 1 = LDS-binned (32x8 source tiles resolve their atomicMin
- * in an LDS bin of the tile's footprint +-16 pixels first, then one HBM
- * atomicMin per non-empty bin cell; targets outside the bin go to HBM).
+ * in an LDS bin first - the source tile moved by its first splatting
+ * pixel's displacement, plus 4 cells a side - then one HBM atomicMin per
+ * non-empty bin cell; targets outside the bin go to HBM).
  * Both leave identical splat buffers. No counterpart in the reference
  * (direct atomicMin, src/renderer.cu:144-145). */
 #define ISTI_AUTOTUNE_TIMES (ISTI_N_TILE_SHAPES + 2)

@@ -44,7 +44,9 @@ def test_counter_summary_covers_every_variant_the_bench_can_run():
 
     c = json.loads((ROOT / "profiles" / "counters_1080p.json").read_text())["kernels"]
     names = ["clear_splat_kernel", "reproject_splat_kernel", "reproject_splat_binned_kernel"]
-    names += [bench.resolve_kernel_name(1, False, 2, t) for t in ((16, 16), (32, 8), (64, 4), (8, 32))]
+    from imagebasedspatiotemporalinterpolation_amd._native import TILE_SHAPES
+
+    names += [bench.resolve_kernel_name(1, False, 2, t) for t in TILE_SHAPES]
     for n in names:
         assert c[n]["traffic"] > 0 and c[n]["valu_insts"] > 0, n
     assert c["clear_splat_kernel"]["traffic"] >= 49_766_400 * 0.99
