@@ -4,10 +4,13 @@ ratios of 1, 2 and 3 (different per axis in one case: the ratio is taken
 per axis, renderer.cu:432-434), and 3 and 4 aux cameras (the scene's two aux
 views repeated; 4 is the largest count the boundary takes). Same bar as the
 other parity cases: splat buffers byte for byte, flows within 1e-3 px, the
-frame within 1/255 and >= 50 dB."""
+frame within 1/255 and >= 50 dB. Every resolve tile shape renders each case
+(the one-wave 8x8 tile stages its halo and aux windows with 64 threads)."""
 import numpy as np
 import pytest
 
+from imagebasedspatiotemporalinterpolation_amd import _native as N
+from imagebasedspatiotemporalinterpolation_amd import renderer as R
 from tests import _frames as F
 
 pytestmark = pytest.mark.gpu
@@ -37,12 +40,17 @@ def test_edge_sizes_and_aux_counts(oracle_lib, w, h, aw, ah, n_aux, b, mode):
     prev = np.random.default_rng(11).integers(0, 256, (th, tw, 4), dtype=np.uint8)
     print(f"case: {w}x{h} aux {aw}x{ah} x{n_aux} b={b} mode={mode}", flush=True)
     ref, sref = F.oracle_frame(oracle_lib, fwd, bwd, vp, ivp, b / fpa, fpa, sfpa, mode, 32, False, prev)
-    got, sgot = F.gpu_frame(fwd, bwd, vp, ivp, b / fpa, fpa, sfpa, mode, 32, False, prev)
-    if mode != 4:
-        for key, s in sref.items():
-            g = sgot[key]
-            np.testing.assert_array_equal(g["splat"], s["splat"], err_msg=f"splat {key}")
-            assert F.flow_err_px(g["cam_ff"], s["cam_ff"]) <= 1e-3, f"cam_ff {key}"
-            assert F.flow_err_px(g["full_ff"], s["full_ff"]) <= 1e-3, f"full_ff {key}"
-            F.assert_visible_flags(g, s, key)
-    F.assert_rgb_parity(got, ref)
+    for shape in range(N.ISTI_N_TILE_SHAPES):
+        print(f"  tile shape {N.TILE_SHAPES[shape]}", flush=True)
+        r = R.Renderer(n_aux=n_aux)
+        r.set_tile_shape(shape)
+        got, sgot = F.gpu_frame(fwd, bwd, vp, ivp, b / fpa, fpa, sfpa, mode, 32, False, prev, renderer=r)
+        if mode != 4:
+            for key, s in sref.items():
+                g = sgot[key]
+                np.testing.assert_array_equal(g["splat"], s["splat"], err_msg=f"splat {key}")
+                assert F.flow_err_px(g["cam_ff"], s["cam_ff"]) <= 1e-3, f"cam_ff {key}"
+                assert F.flow_err_px(g["full_ff"], s["full_ff"]) <= 1e-3, f"full_ff {key}"
+                F.assert_visible_flags(g, s, key)
+        F.assert_rgb_parity(got, ref)
+        r.close()
