@@ -136,3 +136,40 @@ def test_batch_mixes_binned_and_direct_splats(sessions, graph):
         for s in sessions:
             s["r"].set_splat_binning(-1)
         batch.close()
+
+
+def test_full_size_stereo_batch_with_binned_splat():
+    """Both 2064x2208 eyes of the stereo workload in one batch, splatting
+    through the LDS bin (the variant the autotuner picks for this size): every
+    output and splat buffer equals the eye rendered alone with the direct
+    splat, on a pair's first frame (which writes the flow toward the other
+    keyframe) and a later one."""
+    cfg = F.scaled(3, 4)
+    eyes = [_session(cfg, 0, eye=e) for e in (0, 1)]
+    batch = R.Batch(max_requests=2, graph=True)
+    try:
+        for b in (1, 4):
+            infos = [_info(s, b) for s in eyes]
+            prevs = [np.random.default_rng(70 + b + i).integers(0, 256, (s["h"], s["w"], 4), dtype=np.uint8)
+                     for i, s in enumerate(eyes)]
+            expect, splats = [], []
+            for s, info, prev in zip(eyes, infos, prevs):
+                s["r"].set_splat_binning(0)
+                s["r"].set_frame(R.FrameData(s["keep"][0][0], s["keep"][1][0]))
+                expect.append(_alone(s, info, prev))
+                splats.append([s["r"].scratch(sl, 0)["splat"] for sl in (0, 1)])
+                s["r"].set_splat_binning(1)
+                if b == 1:  # the pair anew: the batch writes its flow toward the other keyframe
+                    s["r"].set_frame(R.FrameData(s["keep"][0][0], s["keep"][1][0]))
+            bufs = [torch.from_numpy(p.copy()).cuda() for p in prevs]
+            print(f"stereo 2x{eyes[0]['w']}x{eyes[0]['h']} frame {b}: batched, binned splat", flush=True)
+            batch.render([(s["r"], info, buf) for s, info, buf in zip(eyes, infos, bufs)])
+            torch.cuda.synchronize()
+            for e, (s, buf) in enumerate(zip(eyes, bufs)):
+                np.testing.assert_array_equal(buf.cpu().numpy(), expect[e], err_msg=f"eye {e} frame {b}")
+                for sl in (0, 1):
+                    assert np.array_equal(s["r"].scratch(sl, 0)["splat"], splats[e][sl]), (e, sl, b)
+    finally:
+        batch.close()
+        for s in eyes:
+            s["r"].close()
