@@ -668,6 +668,13 @@ def main():
         res["serving"] = serving.run_mix(duration_s=args.serving_seconds, seed=args.seed, n_streams=3,
                                          max_batch=16, graph=True, store=store)
         res["serving"]["workload"] = mix
+        # the same mix offered 4x faster than it runs at 90 Hz, beyond what one GPU serves:
+        # the queue grows, batches fill up, throughput is the device's (latency is queueing)
+        print("[bench] serving mix at 4x the arrival rate", file=sys.stderr, flush=True)
+        res["serving_overload"] = serving.run_mix(rate_hz=360.0, duration_s=min(1.0, args.serving_seconds),
+                                                  seed=args.seed, n_streams=3, max_batch=16, graph=True,
+                                                  store=store)
+        res["serving_overload"]["workload"] = mix.replace("90 Hz", "360 Hz")
         if args.learned_frames > 0:
             print("[bench] serving mix, learned starts", file=sys.stderr, flush=True)
             res["serving_learned"] = serving.run_mix(duration_s=args.serving_seconds, seed=args.seed, n_streams=3,
