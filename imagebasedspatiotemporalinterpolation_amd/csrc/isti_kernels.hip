@@ -242,7 +242,8 @@ __global__ __launch_bounds__(256) void reproject_splat_binned_kernel(const Splat
 // Learned search starts: the same resolve, with the main camera's searches
 // starting where the network says (see resolve_tile, LEARNED).
 template <int NAUX>
-__global__ __launch_bounds__(256) void resolve_learned_kernel(const ResolveArgs args) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(resolve_min_waves_wide(1, NAUX))))
+void resolve_learned_kernel(const ResolveArgs args) {
 #if defined(__HIP_DEVICE_COMPILE__)
     const KernArg<ResolveArgs> &a = *kernarg_ptr<ResolveArgs>();
 #else
@@ -445,7 +446,7 @@ __global__ __launch_bounds__(256) void splat_batch_kernel(const BatchTable *__re
 }
 
 template <int W, bool MAV, int NAUX>
-__global__ __launch_bounds__(BATCH_TILE_W * BATCH_TILE_H) __attribute__((amdgpu_waves_per_eu(resolve_min_waves(W, NAUX))))
+__global__ __launch_bounds__(BATCH_TILE_W * BATCH_TILE_H) __attribute__((amdgpu_waves_per_eu(resolve_min_waves_wide(W, NAUX))))
 void resolve_batch_kernel(const BatchTable *__restrict__ tg, int variant) {
     __shared__ ResolveLds<W, NAUX, BATCH_TILE_W, BATCH_TILE_H> lds;
     ISTI_CONST_TABLE(t, tg);

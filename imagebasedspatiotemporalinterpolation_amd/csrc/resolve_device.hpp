@@ -58,6 +58,13 @@ __device__ __forceinline__ uint32_t xcd_stripe_tile(uint32_t b, uint32_t n_tiles
 // VGPRs). A floor that forces spills costs more than the waves gain. 1 = no
 // floor.
 constexpr int resolve_min_waves(int w, int naux) { return w != 1 ? 1 : (naux == 2 ? 8 : (naux == 3 ? 6 : 1)); }
+// The same floor for the batched and the learned resolve kernels, whose extra
+// state (the work-table walk, the learned starts and their fallback) needs
+// more registers: for the 2-aux window-1 variant 7 waves (72 VGPRs, an 8-byte
+// spill) measured faster than 8 (44-byte spill) and than no floor (83-90
+// VGPRs, 5 waves): batched 1080p + 2 aux frames 0.276 -> 0.253 ms, learned
+// starts on that workload 7.1 -> 7.9 Gpx/s.
+constexpr int resolve_min_waves_wide(int w, int naux) { return w == 1 && naux == 2 ? 7 : 1; }
 
 // ---------------------------------------------------------------------------
 // K2: resolve (renderer.cu:148-465)
