@@ -464,10 +464,10 @@ void resolve_batch_kernel(const BatchTable *__restrict__ tg, int variant) {
 #else
         const ResolveArgs &a = t->req[q].args;
 #endif
-        // a request's tiles in stripe-major order, so an XCD's contiguous
-        // range of items covers whole vertical stripes (xcd_stripe_tile)
-        resolve_tile<W, MAV, NAUX, false, BATCH_TILE_W, BATCH_TILE_H>(
-            a, stripe_major_tile(base + k - tile_begin[q], a.n_tiles, a.tiles_x), lds);
+        // row-major tiles: an XCD's contiguous range of items is a band of
+        // rows (the stripe-major order of the per-render resolve measured
+        // 270 -> 280 us per batched stereo frame here)
+        resolve_tile<W, MAV, NAUX, false, BATCH_TILE_W, BATCH_TILE_H>(a, base + k - tile_begin[q], lds);
         __syncthreads();  // the next tile restages LDS
     }
 }

@@ -23,7 +23,7 @@ constexpr int N_TILE_SHAPES = 4;
 constexpr uint32_t TILE_SHAPE_W[N_TILE_SHAPES] = {16, 32, 8, 8};
 constexpr uint32_t TILE_SHAPE_H[N_TILE_SHAPES] = {16, 8, 8, 32};
 // Tile shape of the persistent batched resolve (16x16; one-wave 8x8 tiles
-// were slower there: 17.6 against 18.8 Gpx/s on the batched stereo frames).
+// were slower there: 17.6 against 18.3 Gpx/s on the batched stereo frames).
 constexpr int BATCH_TILE_SHAPE = 0;
 constexpr uint32_t BATCH_TILE_W = TILE_SHAPE_W[BATCH_TILE_SHAPE], BATCH_TILE_H = TILE_SHAPE_H[BATCH_TILE_SHAPE];
 
