@@ -48,21 +48,26 @@ __global__ __launch_bounds__(256) void clear_splat_kernel(const ClearArgs a) {
 // float4 flow stores are fully coalesced; the u64 atomicMin is the only
 // scattered access.
 // ---------------------------------------------------------------------------
-// Reprojection of source pixel (x, y) (p = y w + x): writes full_ff and
-// cam_ff and returns whether it splats, with its target pixel and key.
-template <typename J_t, typename M_t>
-__device__ __forceinline__ bool project_pixel(const J_t &J, const M_t &vp_cam, const M_t &vp_main, bool init_linear,
-                                              uint32_t x, uint32_t y, uint32_t p, uint32_t &tx, uint32_t &ty,
-                                              unsigned long long &key) {
+// What a source pixel of a keyframe view contributes to every frame of its
+// keyframe pair: its clip-space position, its world position at the
+// keyframe and moved to the other keyframe, and its visible-in-main flag.
+struct PairPoint {
+    float4 cs_from;
+    float4 ws_from, ws_to;
+    float vis;
+};
 
+template <typename J_t, typename M_t>
+__device__ __forceinline__ PairPoint pair_point(const J_t &J, const M_t &vp_main, uint32_t x, uint32_t y, uint32_t p) {
+    PairPoint q;
     const float z_from = J.z[p];
     float4 om = make_float4(0.f, 0.f, 0.f, 0.f);
     if (J.mx != nullptr) om = make_float4(J.mx[p], J.my[p], J.mz[p], 0.f);
 
-    const float4 cs_from = make_float4(cs_coord(x, J.inv_w), cs_coord(y, J.inv_h), z_from, 1.0f);
-    const float4 ws_from = project_unit(J.inv_vp_from, cs_from);
-    const float4 cs_moved = make_float4(cs_from.x + om.x, cs_from.y + om.y, cs_from.z + om.z, cs_from.w + 0.0f);
-    const float4 ws_to = project_unit(J.inv_vp_from, cs_moved);
+    q.cs_from = make_float4(cs_coord(x, J.inv_w), cs_coord(y, J.inv_h), z_from, 1.0f);
+    q.ws_from = project_unit(J.inv_vp_from, q.cs_from);
+    const float4 cs_moved = make_float4(q.cs_from.x + om.x, q.cs_from.y + om.y, q.cs_from.z + om.z, q.cs_from.w + 0.0f);
+    q.ws_to = project_unit(J.inv_vp_from, cs_moved);
 
     // visible in the main view: -lim < RN32(c / w) < lim for c = x, y, z, with
     // lim = RN32(1 - RN32(1e-3)) = 16760439 * 2^-24. RN32 is monotone and lim's
@@ -76,7 +81,7 @@ __device__ __forceinline__ bool project_pixel(const J_t &J, const M_t &vp_cam, c
     // per job) skips the test and stores 0.
     bool visible = false;
     if (J.main_slot >= 0) {
-        const float4 rm = mat_vec(vp_main, ws_from);
+        const float4 rm = mat_vec(vp_main, q.ws_from);
         if (__builtin_amdgcn_classf(rm.w, CLASS_FINITE_NONZERO)) {
             const double bound = LIM_MID * __builtin_fabs((double)rm.w);
             visible = __builtin_fabs((double)rm.x) <= bound && __builtin_fabs((double)rm.y) <= bound &&
@@ -87,44 +92,69 @@ __device__ __forceinline__ bool project_pixel(const J_t &J, const M_t &vp_cam, c
             visible = qx > -lim && qx < lim && qy > -lim && qy < lim && qz > -lim && qz < lim;
         }
     }
-    const float vis = visible ? 1.0f : 0.0f;
+    q.vis = visible ? 1.0f : 0.0f;
+    return q;
+}
 
+// The flow toward the other keyframe (renderer.cu:119-127 of the splat):
+// cs_to - cs_from.
+template <typename J_t>
+__device__ __forceinline__ float3 full_flow(const J_t &J, const PairPoint &q) {
+    const float4 cs_to = project(J.vp_to, q.ws_to);
+    return make_float3(cs_to.x - q.cs_from.x, cs_to.y - q.cs_from.y, cs_to.z - q.cs_from.z);
+}
+
+// One frame of the pair (weight wt toward the other keyframe, render camera
+// vp_cam): the cam_ff record of the source pixel, and whether it splats, with
+// its target pixel and key.
+template <typename M_t>
+__device__ __forceinline__ bool frame_point(const PairPoint &q, const M_t &vp_cam, float wt, uint32_t x, uint32_t y,
+                                            uint32_t w, uint32_t h, float4 &cam, uint32_t &tx, uint32_t &ty,
+                                            unsigned long long &key) {
     // lerp of the two world points; both w are 1, so the lerped w is
     // 1 + wt * 0: 1 for a finite weight and NaN (carried in x) otherwise
-    const float wt = J.weight;
-    const float sx = ws_from.x + wt * (ws_to.x - ws_from.x);
+    const float sx = q.ws_from.x + wt * (q.ws_to.x - q.ws_from.x);
     const float4 ws_scaled = make_float4(__builtin_isfinite(wt) ? sx : __builtin_nanf(""),
-                                         ws_from.y + wt * (ws_to.y - ws_from.y),
-                                         ws_from.z + wt * (ws_to.z - ws_from.z), 1.0f);
+                                         q.ws_from.y + wt * (q.ws_to.y - q.ws_from.y),
+                                         q.ws_from.z + wt * (q.ws_to.z - q.ws_from.z), 1.0f);
+    const float4 cs_cam = project(vp_cam, ws_scaled);
+    cam = make_float4(cs_cam.x - q.cs_from.x, cs_cam.y - q.cs_from.y, q.cs_from.z + (cs_cam.z - q.cs_from.z), q.vis);
+    const float depth_cam = z_to_depth(cs_cam.z);
+    const int2 pc = cs2pixel(cs_cam.x, cs_cam.y, w, h);
+    if (!in_range(pc, w, h) || depth_cam < 0.0f) return false;
+    tx = (uint32_t)pc.x;
+    ty = (uint32_t)pc.y;
+    key = encode_key(depth_cam, x, y);
+    return true;
+}
+
+// Reprojection of source pixel (x, y) (p = y w + x): writes full_ff and
+// cam_ff and returns whether it splats, with its target pixel and key.
+template <typename J_t, typename M_t>
+__device__ __forceinline__ bool project_pixel(const J_t &J, const M_t &vp_cam, const M_t &vp_main, bool init_linear,
+                                              uint32_t x, uint32_t y, uint32_t p, uint32_t &tx, uint32_t &ty,
+                                              unsigned long long &key) {
+    const PairPoint q = pair_point(J, vp_main, x, y, p);
+    const float wt = J.weight;
     // The flow toward the other keyframe depends on the keyframe pair alone
     // (not on the render camera or alpha): the first render of a pair writes
     // it and later renders keep it (write_full is uniform per job). The
     // linear initialisation scales it into cam_ff, so it computes it anyway.
-    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    float3 f = make_float3(0.0f, 0.0f, 0.0f);
     if (J.write_full || init_linear) {
-        const float4 cs_to = project(J.vp_to, ws_to);
-        fx = cs_to.x - cs_from.x;
-        fy = cs_to.y - cs_from.y;
-        fz = cs_to.z - cs_from.z;
-        if (J.write_full) J.full_ff[p] = Flow3{fx, fy, fz};
+        f = full_flow(J, q);
+        if (J.write_full) J.full_ff[p] = Flow3{f.x, f.y, f.z};
     }
-
-    float depth = z_to_depth(cs_from.z);
+    if (!init_linear) {
+        float4 cam;
+        const bool hit = frame_point(q, vp_cam, wt, x, y, J.w, J.h, cam, tx, ty, key);
+        J.cam_ff[p] = cam;
+        return hit;
+    }
+    J.cam_ff[p] = make_float4(f.x * wt, f.y * wt, q.cs_from.z + f.z * wt, q.vis);
     tx = x;
     ty = y;
-    if (!init_linear) {
-        const float4 cs_cam = project(vp_cam, ws_scaled);
-        J.cam_ff[p] = make_float4(cs_cam.x - cs_from.x, cs_cam.y - cs_from.y, z_from + (cs_cam.z - cs_from.z), vis);
-        const float depth_cam = z_to_depth(cs_cam.z);
-        const int2 pc = cs2pixel(cs_cam.x, cs_cam.y, J.w, J.h);
-        if (!in_range(pc, J.w, J.h) || depth_cam < 0.0f) return false;
-        depth = depth_cam;
-        tx = (uint32_t)pc.x;
-        ty = (uint32_t)pc.y;
-    } else {
-        J.cam_ff[p] = make_float4(fx * wt, fy * wt, z_from + fz * wt, vis);
-    }
-    key = encode_key(depth, x, y);
+    key = encode_key(z_to_depth(q.cs_from.z), x, y);
     return true;
 }
 

@@ -92,3 +92,41 @@ def test_frames_argument_checks():
     torch.cuda.synchronize()
     assert torch.equal(one, two)
     r.close()
+
+
+def test_frames_beyond_one_set_and_with_linear_initialisation():
+    """11 frames in one call (the two streams and buffer sets each reused
+    several times), then a call with a linear-initialisation frame among
+    them, then a plain render after both: every image equals isti_render
+    frame by frame."""
+    cfg = F.scaled(2, 24, 4)
+    fd, n_aux, infos, keep = _setup(cfg)
+    seq = R.Renderer(n_aux=n_aux)
+    seq.set_frame(fd)
+    many = R.Renderer(n_aux=n_aux)
+    many.set_frame(fd)
+    calls = [[infos[i % len(infos)] for i in range(11)]]
+    lin = R.RenderInfo(infos[2].resolution, infos[2].alpha, infos[2].viewproj_mat, infos[2].inv_viewproj_mat,
+                       R.AdditionalRenderInfo(cfg.keyframe_spacing, cfg.shadow_frames, False,
+                                              R.RenderMode.BACKWARD_BIDIRECTIONAL), -1)
+    calls.append([infos[0], lin, infos[4]])
+    for c, frame_infos in enumerate(calls):
+        print(f"case: call {c}, {len(frame_infos)} frames", flush=True)
+        prevs = _prevs(frame_infos, 40 + c)
+        a = [torch.from_numpy(p.copy()).cuda() for p in prevs]
+        for buf, info in zip(a, frame_infos):
+            seq.render(buf, info)
+        b = [torch.from_numpy(p.copy()).cuda() for p in prevs]
+        many.render_frames(b, frame_infos)
+        torch.cuda.synchronize()
+        for i, (x, y) in enumerate(zip(a, b)):
+            np.testing.assert_array_equal(y.cpu().numpy(), x.cpu().numpy(), err_msg=f"call {c} frame {i}")
+    prev = _prevs(infos[:1], 99)[0]
+    x = torch.from_numpy(prev.copy()).cuda()
+    y = torch.from_numpy(prev.copy()).cuda()
+    seq.render(x, infos[3])
+    many.render(y, infos[3])
+    torch.cuda.synchronize()
+    assert torch.equal(x, y)
+    seq.close()
+    many.close()
