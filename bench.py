@@ -629,6 +629,12 @@ def main():
             "algorithmic_gbps": round(alg["splat"] / (stage_avg_ms["splat"] * 1e-3) / 1e9, 1),
             "traffic_gbps": (round(counter("splat", "traffic") / (stage_avg_ms["splat"] * 1e-3) / 1e9, 1)
                              if counter("splat", "traffic") is not None else None),
+            # every atomicMin is performed past the L2; the direct splat completes those
+            # requests (TCC_ATOMIC_sum per launch, from the counter file) at ~18.9 G/s on
+            # this and on the stereo workload, which is the limit it sits on
+            "l2_atomic_requests_per_launch": counter("splat", "l2_atomics"),
+            "atomic_requests_per_s": (round(counter("splat", "l2_atomics") / (stage_avg_ms["splat"] * 1e-3), 1)
+                                      if counter("splat", "l2_atomics") is not None else None),
         },
         # achieved/peak/frac: algorithmic HBM bytes against the HBM peak; the
         # VALU-issue fraction beside it; `bound` names the larger of the two
