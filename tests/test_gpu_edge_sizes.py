@@ -5,7 +5,9 @@ per axis, renderer.cu:432-434), and 3 and 4 aux cameras (the scene's two aux
 views repeated; 4 is the largest count the boundary takes). Same bar as the
 other parity cases: splat buffers byte for byte, flows within 1e-3 px, the
 frame within 1/255 and >= 50 dB. Every resolve tile shape renders each case
-(the one-wave 8x8 tile stages its halo and aux windows with 64 threads)."""
+(the one-wave 8x8 tile stages its halo and aux windows with 64 threads), and
+every other one splats through the LDS bin (views narrower or shorter than
+its 32x8 source tile included)."""
 import numpy as np
 import pytest
 
@@ -41,9 +43,11 @@ def test_edge_sizes_and_aux_counts(oracle_lib, w, h, aw, ah, n_aux, b, mode):
     print(f"case: {w}x{h} aux {aw}x{ah} x{n_aux} b={b} mode={mode}", flush=True)
     ref, sref = F.oracle_frame(oracle_lib, fwd, bwd, vp, ivp, b / fpa, fpa, sfpa, mode, 32, False, prev)
     for shape in range(N.ISTI_N_TILE_SHAPES):
-        print(f"  tile shape {N.TILE_SHAPES[shape]}", flush=True)
+        binned = shape % 2  # the LDS-binned splat on every other shape
+        print(f"  tile shape {N.TILE_SHAPES[shape]} splat {'binned' if binned else 'direct'}", flush=True)
         r = R.Renderer(n_aux=n_aux)
         r.set_tile_shape(shape)
+        r.set_splat_binning(binned)
         got, sgot = F.gpu_frame(fwd, bwd, vp, ivp, b / fpa, fpa, sfpa, mode, 32, False, prev, renderer=r)
         if mode != 4:
             for key, s in sref.items():
