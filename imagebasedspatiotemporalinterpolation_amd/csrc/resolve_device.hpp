@@ -450,6 +450,23 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
     const uint32_t x0 = tx * TW, y0 = ty * TH;
     const uint32_t x = x0 + (threadIdx.x & (TW - 1));
     const uint32_t y = y0 + (threadIdx.x / TW);
+    // Learned starts: the network's output at this pixel's cell and cam_ff.xy
+    // at the pixel depend on the pixel alone, so they are loaded before the
+    // staging and its barriers, which hide their latency (loaded after them,
+    // every wave waited for them at the start of its searches).
+    float net_o[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+    float2 net_m[2] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+    if (LEARNED && x < a.w && y < a.h) {
+#pragma unroll
+        for (int d = 0; d < 2; d++) {
+            const ResolveView &v = d ? a.bwd[0] : a.fwd[0];
+            const float *o = a.net_out + (((size_t)d * a.net_h + (y >> a.net_shift)) * a.net_w + (x >> a.net_shift)) * 16;
+            net_o[d][0] = o[0];
+            net_o[d][1] = o[1];
+            net_o[d][2] = o[2];
+            net_m[d] = at_xy(v.cam_ff, v.w, x, y);
+        }
+    }
 
     // Stage every splat cell the tile's windows read: camera c covers camera
     // pixels [x0/rx, xl/rx] (xl = last in-image column of the tile) plus a
@@ -566,12 +583,12 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
             for (int d = 0; d < 2; d++) {
                 if (d == 0 && !(fc.alpha < 1.0f)) continue;
                 const ResolveView &v = d ? a.bwd[0] : a.fwd[0];
-                const float *o = a.net_out + (((size_t)d * a.net_h + (y >> a.net_shift)) * a.net_w + (x >> a.net_shift)) * 16;
+                const float *o = net_o[d];
                 if (o[2] >= a.conf_logit) {
                     // one analytic search step from the pixel itself (p - cam_ff(p)),
                     // corrected by the network's residual in cells of 2^net_shift pixels
                     const float cell = (float)(1u << a.net_shift);
-                    const float4 m = at(v.cam_ff, v.w, x, y);
+                    const float2 m = net_m[d];
                     float fx = ((float)x + 0.5f - m.x * (0.5f * (float)v.w)) + o[0] * cell;
                     float fy = ((float)y + 0.5f - m.y * (0.5f * (float)v.h)) + o[1] * cell;
                     fx = fx >= 0.0f ? (fx <= (float)(v.w - 1) ? fx : (float)(v.w - 1)) : 0.0f;  // NaN -> 0
