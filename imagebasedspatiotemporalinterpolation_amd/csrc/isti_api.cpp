@@ -50,6 +50,11 @@ struct Scratch {
     // on the pair alone); cleared when either view of the pair is set, on a
     // swap and when the buffers are reallocated
     bool full_ok = false;
+    // second splat buffer of the view: the plain resolve of one frame clears
+    // it (ClearAhead) while it reads `splat`, and the next frame planned on
+    // the view swaps it in and skips its clear
+    unsigned long long *spare = nullptr;
+    bool spare_clean = false;
 };
 
 // Ring of per-render event sets for stage timing. A slot is harvested (its
@@ -108,6 +113,7 @@ isti_status dev_fail(isti_renderer *r, hipError_t e, const char *where) {
 
 void free_scratch(Scratch &s) {
     if (s.splat) (void)hipFree(s.splat);
+    if (s.spare) (void)hipFree(s.spare);
     if (s.cam_ff) (void)hipFree(s.cam_ff);
     if (s.full_ff) (void)hipFree(s.full_ff);
     s = Scratch{};
@@ -132,6 +138,7 @@ isti_status ensure_scratch(isti_renderer *r, Scratch &s, uint32_t w, uint32_t h)
 // The second buffer set of isti_render_frames: splat and cam_ff only.
 void free_scratch2(Scratch &s) {
     if (s.splat) (void)hipFree(s.splat);
+    if (s.spare) (void)hipFree(s.spare);
     if (s.cam_ff) (void)hipFree(s.cam_ff);
     s = Scratch{};
 }
@@ -325,14 +332,30 @@ struct FramePlan {
     int window = 1;
     bool always_valid = false;
     bool own_scratch = true;  // false: planes carved from caller-provided scratch
+    // job j's splat buffer already holds the empty key (a resolve cleared it
+    // ahead), so its clear is skipped; reset once the buffer is splatted
+    bool clean[MAX_SPLAT_JOBS] = {};
 };
+
+// The view's spare splat buffer, once a resolve has cleared it (ClearAhead),
+// becomes the buffer of the next frame planned on the view; returns whether
+// that happened (the frame's clear of this buffer is then skipped).
+bool take_clean(Scratch &s) {
+    if (!s.spare || !s.spare_clean) return false;
+    std::swap(s.splat, s.spare);
+    s.spare_clean = false;
+    return true;
+}
 
 // ext: per (slot, view) scratch planes carved from caller-provided memory, or
 // null for the renderer's own. Carved planes may hold another call's data, so
 // the flow toward the other keyframe is always recomputed, unless ext_full
 // says that ext's full_ff planes (and full_ok flags) are the renderer's own.
+// The renderer's own views swap in a cleared spare splat buffer (take_clean);
+// bit (slot * MAX_VIEWS + view) of ext_clean says the same of ext's buffers.
 isti_status plan_frame(isti_renderer *r, const isti_render_info *info, uint8_t *out, size_t out_pitch,
-                       FramePlan &P, const Scratch (*ext)[MAX_VIEWS] = nullptr, bool ext_full = false) {
+                       FramePlan &P, const Scratch (*ext)[MAX_VIEWS] = nullptr, bool ext_full = false,
+                       uint32_t ext_clean = 0) {
     const int nv = 1 + r->n_aux;
     Scratch (*scratch)[MAX_VIEWS] = r->scratch;
     if (ext) {
@@ -355,6 +378,7 @@ isti_status plan_frame(isti_renderer *r, const isti_render_info *info, uint8_t *
             const isti_view_desc &d = r->views[sl][v];
             const isti_view_desc &other = r->views[1 - sl][v];
             Scratch &sc = scratch[sl][v];
+            P.clean[P.nj] = ext ? ((ext_clean >> (sl * MAX_VIEWS + v)) & 1u) != 0 : take_clean(sc);
             SplatJob &J = P.jobs[P.nj++];
             const bool hm = has_motion(d);
             J.z = d.z;
@@ -442,7 +466,7 @@ void mark_full_written(isti_renderer *r) {
         for (int v = 0; v <= r->n_aux; v++) r->scratch[sl][v].full_ok = true;
 }
 
-isti_status splat_stage(isti_renderer *r, const FramePlan &P, hipStream_t s, bool binned) {
+isti_status splat_stage(isti_renderer *r, FramePlan &P, hipStream_t s, bool binned) {
     ClearArgs ca;
     std::memset(&ca, 0, sizeof(ca));
     SplatArgs sa;
@@ -452,29 +476,96 @@ isti_status splat_stage(isti_renderer *r, const FramePlan &P, hipStream_t s, boo
     sa.vp_main[1] = P.vp_main[1];
     sa.init_linear = P.init_linear;
     uint32_t clear_blocks = 0, n_splat_blocks = 0;
+    int nc = 0;
     for (int j = 0; j < P.nj; j++) {
         SplatJob &J = sa.job[j];
         J = P.jobs[j];
-        ca.buf[j] = J.splat;
-        ca.n[j] = J.n_px;
-        ca.block_begin[j] = clear_blocks;
-        clear_blocks += (J.n_px + 511u) / 512u;
+        if (!P.clean[j]) {
+            ca.buf[nc] = J.splat;
+            ca.n[nc] = J.n_px;
+            ca.block_begin[nc] = clear_blocks;
+            clear_blocks += (J.n_px + 511u) / 512u;
+            nc++;
+        }
         J.block_begin = n_splat_blocks;
         J.binned = binned ? 1 : 0;
         n_splat_blocks += splat_blocks(J.w, J.h, binned);
     }
-    ca.n_bufs = P.nj;
+    ca.n_bufs = nc;
     ca.n_blocks = clear_blocks;
     sa.n_jobs = P.nj;
     sa.n_blocks = n_splat_blocks;
     hipError_t e;
     if ((e = mark(r, 0, s)) != hipSuccess) return dev_fail(r, e, "event");
-    if ((e = launch_clear(ca, s)) != hipSuccess) return dev_fail(r, e, "clear launch");
+    if (clear_blocks && (e = launch_clear(ca, s)) != hipSuccess) return dev_fail(r, e, "clear launch");
     if ((e = mark(r, 1, s)) != hipSuccess) return dev_fail(r, e, "event");
     if ((e = launch_splat(sa, binned, s)) != hipSuccess) return dev_fail(r, e, "splat launch");
+    for (int j = 0; j < P.nj; j++) P.clean[j] = false;
     if (P.own_scratch) mark_full_written(r);
     if ((e = mark(r, 2, s)) != hipSuccess) return dev_fail(r, e, "event");
     return ISTI_OK;
+}
+
+// Points a plain resolve launch of tile shape `shape` at the spare splat
+// buffers of the views in `set` that are not clear yet, allocating them on
+// first use. The launch clears them after its own work (ClearAhead); call
+// ahead_cleared once it is enqueued. The spare of a view was last splatted by
+// an earlier frame on the same stream, whose resolve precedes this one.
+isti_status plan_ahead(isti_renderer *r, Scratch (*set)[MAX_VIEWS], ResolveArgs &ra, int shape) {
+    ClearAhead &c = ra.ahead;
+    std::memset(&c, 0, sizeof(c));
+    for (int sl = 0; sl < 2; sl++)
+        for (int v = 0; v <= r->n_aux; v++) {
+            Scratch &sc = set[sl][v];
+            if (sc.spare_clean || !sc.splat) continue;
+            const size_t n = (size_t)sc.w * sc.h;
+            if (!sc.spare) {
+                const hipError_t e = hipMalloc(&sc.spare, n * sizeof(unsigned long long));
+                if (e != hipSuccess) {
+                    sc.spare = nullptr;
+                    return dev_fail(r, e, "hipMalloc(spare splat buffer)");
+                }
+            }
+            c.buf[c.n_bufs] = sc.spare;
+            c.n[c.n_bufs] = (uint32_t)n;
+            c.n_bufs++;
+        }
+    // fewest runs per workgroup that fit the clear into the resolve grid; a
+    // grid with fewer workgroups than buffers (each buffer takes at least
+    // one), or one that would need more than MAX_ROUNDS runs per workgroup
+    // (a small render of large keyframes), leaves the clear to the next
+    // frame's clear kernel
+    constexpr uint64_t MAX_ROUNDS = 16;
+    const uint64_t per_round = 2ull * TILE_SHAPE_W[shape] * TILE_SHAPE_H[shape];  // cells
+    uint64_t cells = 0;
+    for (int j = 0; j < c.n_bufs; j++) cells += c.n[j];
+    uint64_t rounds = c.n_bufs && (uint64_t)c.n_bufs <= ra.n_tiles
+                          ? (cells + per_round * ra.n_tiles - 1) / (per_round * ra.n_tiles)
+                          : MAX_ROUNDS + 1;
+    for (; rounds <= MAX_ROUNDS; rounds++) {
+        uint64_t blocks = 0;
+        for (int j = 0; j < c.n_bufs; j++) blocks += (c.n[j] + per_round * rounds - 1) / (per_round * rounds);
+        if (blocks <= ra.n_tiles) break;
+    }
+    if (rounds > MAX_ROUNDS) {
+        std::memset(&c, 0, sizeof(c));
+        return ISTI_OK;
+    }
+    uint32_t b = 0;
+    for (int j = 0; j < c.n_bufs; j++) {
+        c.block_begin[j] = b;
+        b += (uint32_t)((c.n[j] + per_round * rounds - 1) / (per_round * rounds));
+    }
+    c.n_blocks = b;
+    c.rounds = (uint32_t)rounds;
+    return ISTI_OK;
+}
+
+void ahead_cleared(isti_renderer *r, Scratch (*set)[MAX_VIEWS], const ResolveArgs &ra) {
+    if (ra.ahead.n_bufs == 0) return;
+    for (int sl = 0; sl < 2; sl++)
+        for (int v = 0; v <= r->n_aux; v++)
+            if (set[sl][v].spare) set[sl][v].spare_clean = true;
 }
 
 // Smallest power-of-two reduction of the render size that fits the network grid.
@@ -994,8 +1085,10 @@ isti_status isti_internal_render(isti_renderer *r, const isti_render_info *info,
     r->prepared_valid = false;
     st = splat_stage(r, P, s, splat_binned_for(r, P));
     if (st != ISTI_OK) return st;
+    if (!base && (st = plan_ahead(r, r->scratch, P.ra, shape)) != ISTI_OK) return st;
     e = launch_resolve(P.ra, P.window, P.always_valid, shape, s);
     if (e != hipSuccess) return dev_fail(r, e, "resolve launch");
+    if (!base) ahead_cleared(r, r->scratch, P.ra);
     if ((e = mark(r, 3, s)) != hipSuccess) return dev_fail(r, e, "event");
     return ISTI_OK;
 }
@@ -1046,15 +1139,17 @@ isti_status isti_render_frames(isti_renderer *r, const isti_render_info *infos, 
             st = plan_frame(r, &infos[i], outs[i], pitches[i], P);
         } else {
             Scratch set2[2][MAX_VIEWS];
+            uint32_t clean = 0;
             for (int sl = 0; sl < 2 && st == ISTI_OK; sl++)
                 for (int v = 0; v <= r->n_aux && st == ISTI_OK; v++) {
                     Scratch &own = r->scratch[sl][v], &b = r->scratch2[sl][v];
                     st = ensure_scratch2(r, b, own.w, own.h);
+                    if (st == ISTI_OK && take_clean(b)) clean |= 1u << (sl * MAX_VIEWS + v);
                     set2[sl][v] = b;
                     set2[sl][v].full_ff = own.full_ff;
                     set2[sl][v].full_ok = own.full_ok;
                 }
-            if (st == ISTI_OK) st = plan_frame(r, &infos[i], outs[i], pitches[i], P, set2, true);
+            if (st == ISTI_OK) st = plan_frame(r, &infos[i], outs[i], pitches[i], P, set2, true, clean);
         }
         if (st != ISTI_OK) break;
         if (i == 0)
@@ -1072,8 +1167,12 @@ isti_status isti_render_frames(isti_renderer *r, const isti_render_info *infos, 
             st = dev_fail(r, e, "hipStreamWaitEvent");
             break;
         }
+        Scratch (*set)[MAX_VIEWS] = k == 0 ? r->scratch : r->scratch2;
+        if ((st = plan_ahead(r, set, P.ra, shape)) != ISTI_OK) break;
         if ((e = launch_resolve(P.ra, P.window, P.always_valid, shape, r->pipe[k])) != hipSuccess)
             st = dev_fail(r, e, "resolve launch");
+        else
+            ahead_cleared(r, set, P.ra);
     }
     r->timing = timing;
     // join (also after an error, so the caller's stream never runs ahead of enqueued work)

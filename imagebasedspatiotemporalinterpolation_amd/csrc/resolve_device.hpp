@@ -436,6 +436,36 @@ struct ResolveLds {
     unsigned long long splat[NC][2][SPAN * ROWS];
 };
 
+// The clear a resolve workgroup of NTH threads does for the next frame
+// (ClearAhead): one 16-byte store per cell pair, non-temporal because the
+// buffer is not read before the next frame's splat. resolve_tile issues it
+// right after its staging loads, so the stores' acknowledgements overlap the
+// latency of those loads, which the workgroup waits for at its first barrier
+// anyway (issued at the end of the tile instead, the workgroup would wait for
+// them before it retires; at the start, they would delay the staging loads).
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+
+template <int NTH, typename A>
+__device__ __forceinline__ void clear_ahead(const A &a) {
+    const uint32_t b = blockIdx.x;
+    if (b >= a.ahead.n_blocks) return;
+    int j = 0;
+    for (int i = 1; i < a.ahead.n_bufs; i++)
+        if (b >= a.ahead.block_begin[i]) j = i;
+    unsigned long long *buf = a.ahead.buf[j];
+    const uint32_t n = a.ahead.n[j];
+    uint32_t pair = (b - a.ahead.block_begin[j]) * a.ahead.rounds * NTH + threadIdx.x;
+    for (uint32_t r = 0; r < a.ahead.rounds; r++, pair += NTH) {
+        const uint32_t i0 = pair * 2u;
+        if (i0 + 1u < n) {
+            const u64x2 e = {SPLAT_EMPTY, SPLAT_EMPTY};
+            __builtin_nontemporal_store(e, reinterpret_cast<u64x2 *>(buf) + pair);
+        } else if (i0 < n) {
+            __builtin_nontemporal_store(SPLAT_EMPTY, buf + i0);
+        }
+    }
+}
+
 // One TW x TH tile of one render, one thread per pixel (TW * TH threads: 64,
 // one wave, or up to 256). Every thread of the block must call it (the LDS
 // staging ends in a barrier); pixels outside the image return early.
@@ -484,6 +514,7 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
         stage_splat<SPAN, NTH>(s_splat[c][0], a.fwd[c], org_x[c], org_y[c], span_x[c], span_y[c]);
         stage_splat<SPAN, NTH>(s_splat[c][1], a.bwd[c], org_x[c], org_y[c], span_x[c], span_y[c]);
     }
+    if constexpr (!LEARNED) clear_ahead<NTH>(a);  // none unless a plain render asked for it
     __syncthreads();
     // 1a. aux cameras: a window start depends only on the aux pixel, which
     // ratio_x * ratio_y render pixels share, so each distinct aux pixel of the

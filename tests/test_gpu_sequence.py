@@ -87,6 +87,53 @@ def test_sequence_of_plain_renders(oracle_lib):
     r.close()
 
 
+def _splat_ptrs(r, n_aux):
+    ptrs = []
+    for sl in (0, 1):
+        for v in range(n_aux + 1):
+            sv = N.ScratchView()
+            N.check(N.lib().isti_scratch(r._h, sl, v, ctypes.byref(sv)), r._h)
+            ptrs.append(sv.splat)
+    return ptrs
+
+
+def test_splat_buffers_cleared_ahead_by_the_resolve(oracle_lib):
+    """A plain resolve clears each view's spare splat buffer after its own
+    work and the next frame splats into it without a clear of its own: the
+    buffers alternate from frame to frame, and every frame's splat buffers
+    still hold exactly the oracle's keys (a stale key left by an incomplete
+    clear would show there), also around isti_render_frames, which clears
+    ahead on its second buffer set as well, and a keyframe size change."""
+    cfg = F.scaled(2, 11, 4)
+    r = R.Renderer(n_aux=2)
+    S = _Seq(oracle_lib, cfg, r)
+    seen = []
+    for i, b in enumerate((1, 2, 3, 4)):
+        print(f"case: plain frame {i} b={b}", flush=True)
+        S.check(S.render(b), b, what=f"plain {i}")
+        seen.append(_splat_ptrs(r, 2))
+    assert all(p != q for p, q in zip(seen[0], seen[1])), "second frame reused the first frame's buffers"
+    assert seen[2] == seen[0] and seen[3] == seen[1], "buffers do not alternate"
+
+    print("case: render_frames, then plain", flush=True)
+    infos = [S.info(b)[0] for b in (1, 2, 3, 4, 5)]
+    w, h = S.info(1)[3:5]
+    outs = [torch.zeros((h, w, 4), dtype=torch.uint8, device="cuda") for _ in infos]
+    r.render_frames(outs, infos)
+    torch.cuda.synchronize()
+    for b, out in zip((1, 2, 3, 4, 5), outs):
+        ref = torch.zeros_like(out)
+        r.render(ref, S.info(b)[0])
+        S.check(ref, b, what=f"plain after render_frames, b={b}")
+        assert torch.equal(out, ref), b
+
+    print("case: keyframe size change", flush=True)
+    S2 = _Seq(oracle_lib, F.scaled(2, 12, 5), r)
+    for b in (1, 2, 3):
+        S2.check(S2.render(b), b, what=f"new size {b}")
+    r.close()
+
+
 def test_sequence_across_paths_and_sizes(oracle_lib):
     cfg = F.scaled(2, 8, 4)
     r = R.Renderer(n_aux=2)
