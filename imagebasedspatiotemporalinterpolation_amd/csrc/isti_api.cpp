@@ -753,6 +753,10 @@ isti_status isti_render_batch(isti_batch *b, isti_renderer *const *rs, const ist
         std::memset(&R, 0, sizeof(R));
         R.args = P.ra;
         set_tiles(R.args, BATCH_TILE_SHAPE);  // the batched resolve's tile grid
+        // (no clear ahead here: clearing the next batch's buffers in the
+        // batched resolve measured slower, 18.3 -> 15.6 Gpx/s on the stereo
+        // frames, whose splat atomics then miss the buffers in the caches a
+        // clear kernel right before the splat leaves them in)
         R.vp_cam = P.vp_cam;
         R.vp_main[0] = P.vp_main[0];
         R.vp_main[1] = P.vp_main[1];
@@ -764,7 +768,7 @@ isti_status isti_render_batch(isti_batch *b, isti_renderer *const *rs, const ist
             jobs[j] = P.jobs[k];
             jobs[j].req = q;
             clear_begin[j] = cb;
-            cb += (P.jobs[k].n_px + 511u) / 512u;
+            if (!P.clean[k]) cb += (P.jobs[k].n_px + 511u) / 512u;  // cleared ahead: no blocks
             splat_begin[j] = sb;
             sb += splat_blocks(P.jobs[k].w, P.jobs[k].h, P.jobs[k].binned != 0);
         }

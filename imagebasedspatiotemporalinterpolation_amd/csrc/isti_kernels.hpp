@@ -84,11 +84,11 @@ struct ResolveView {
     uint32_t ratio_x, ratio_y, magic_x, magic_y;
 };
 
-// Splat buffers a resolve launch fills with the empty key after its own work
+// Splat buffers a resolve launch fills with the empty key beside its own work
 // (the spare buffers of the frame's views, which the next frame of the same
-// stream splats into instead of clearing): workgroup b < n_blocks clears
-// `rounds` runs of one cell pair per thread of buffer j, the last j with
-// block_begin[j] <= b.
+// stream splats into instead of clearing): the workgroup resolving tile
+// b < n_blocks of the render clears `rounds` runs of one cell pair per thread
+// of buffer j, the last j with block_begin[j] <= b.
 struct ClearAhead {
     unsigned long long *buf[MAX_SPLAT_JOBS];
     uint32_t n[MAX_SPLAT_JOBS];
@@ -113,7 +113,7 @@ struct ResolveArgs {
     uint32_t net_w, net_h, net_shift;
     float conf_logit;         // starts are used where the confidence logit >= this
     uint32_t *learned_stats;  // kept, rejected, low confidence, total
-    ClearAhead ahead;         // plain resolve launches only; n_blocks 0 = none
+    ClearAhead ahead;         // per-render resolves only; n_blocks 0 = none
 };
 
 struct ClearArgs {

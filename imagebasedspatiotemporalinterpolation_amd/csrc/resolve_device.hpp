@@ -445,9 +445,9 @@ struct ResolveLds {
 // them before it retires; at the start, they would delay the staging loads).
 typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
 
+// b is the tile's index in its render (each index runs once per launch).
 template <int NTH, typename A>
-__device__ __forceinline__ void clear_ahead(const A &a) {
-    const uint32_t b = blockIdx.x;
+__device__ __forceinline__ void clear_ahead(const A &a, uint32_t b) {
     if (b >= a.ahead.n_blocks) return;
     int j = 0;
     for (int i = 1; i < a.ahead.n_bufs; i++)
@@ -468,8 +468,11 @@ __device__ __forceinline__ void clear_ahead(const A &a) {
 
 // One TW x TH tile of one render, one thread per pixel (TW * TH threads: 64,
 // one wave, or up to 256). Every thread of the block must call it (the LDS
-// staging ends in a barrier); pixels outside the image return early.
-template <int W, bool MAV, int NAUX, bool LEARNED = false, int TW = TILE_W, int TH = TILE_H, typename A>
+// staging ends in a barrier); pixels outside the image return early. AHEAD:
+// the kernel clears its share of a.ahead (only the per-render kernel; the
+// batched and learned kernels are built without that code).
+template <int W, bool MAV, int NAUX, bool LEARNED = false, int TW = TILE_W, int TH = TILE_H, bool AHEAD = false,
+          typename A>
 __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveLds<W, NAUX, TW, TH> &lds) {
     constexpr int NTH = TW * TH;  // threads of the block
     static_assert(NTH % 64 == 0 && NTH <= 256 && (TW & (TW - 1)) == 0, "tile shape");
@@ -514,7 +517,7 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
         stage_splat<SPAN, NTH>(s_splat[c][0], a.fwd[c], org_x[c], org_y[c], span_x[c], span_y[c]);
         stage_splat<SPAN, NTH>(s_splat[c][1], a.bwd[c], org_x[c], org_y[c], span_x[c], span_y[c]);
     }
-    if constexpr (!LEARNED) clear_ahead<NTH>(a);  // none unless a plain render asked for it
+    if constexpr (AHEAD) clear_ahead<NTH>(a, tile);
     __syncthreads();
     // 1a. aux cameras: a window start depends only on the aux pixel, which
     // ratio_x * ratio_y render pixels share, so each distinct aux pixel of the

@@ -17,7 +17,7 @@ void resolve_kernel(const ResolveArgs args) {
     const ResolveArgs &a = args;  // host pass only parses the body
 #endif
     __shared__ ResolveLds<W, NAUX, TW, TH> lds;
-    resolve_tile<W, MAV, NAUX, false, TW, TH>(a, xcd_stripe_tile(blockIdx.x, a.n_tiles, a.tiles_x), lds);
+    resolve_tile<W, MAV, NAUX, false, TW, TH, true>(a, xcd_stripe_tile(blockIdx.x, a.n_tiles, a.tiles_x), lds);
 }
 
 template <int W, bool MAV, int TW, int TH>

@@ -98,12 +98,14 @@ def _splat_ptrs(r, n_aux):
 
 
 def test_splat_buffers_cleared_ahead_by_the_resolve(oracle_lib):
-    """A plain resolve clears each view's spare splat buffer after its own
+    """A plain resolve clears each view's spare splat buffer beside its own
     work and the next frame splats into it without a clear of its own: the
     buffers alternate from frame to frame, and every frame's splat buffers
     still hold exactly the oracle's keys (a stale key left by an incomplete
     clear would show there), also around isti_render_frames, which clears
-    ahead on its second buffer set as well, and a keyframe size change."""
+    ahead on its second buffer set as well, batched frames (the first of which
+    takes the cleared spare; a batch clears its other buffers itself) and a
+    keyframe size change."""
     cfg = F.scaled(2, 11, 4)
     r = R.Renderer(n_aux=2)
     S = _Seq(oracle_lib, cfg, r)
@@ -126,6 +128,17 @@ def test_splat_buffers_cleared_ahead_by_the_resolve(oracle_lib):
         r.render(ref, S.info(b)[0])
         S.check(ref, b, what=f"plain after render_frames, b={b}")
         assert torch.equal(out, ref), b
+
+    print("case: batched frames", flush=True)
+    batch = R.Batch(max_requests=2)
+    for b in (2, 3, 4):
+        ri, _, _, w, h = S.info(b)
+        out = torch.zeros((h, w, 4), dtype=torch.uint8, device="cuda")
+        batch.render([(r, ri, out)])
+        S.check(out, b, what=f"batch {b}")
+    S.check(S.render(5), 5, what="plain after batch")
+    S.check(S.render(1), 1, what="plain after plain after batch")
+    batch.close()
 
     print("case: keyframe size change", flush=True)
     S2 = _Seq(oracle_lib, F.scaled(2, 12, 5), r)
