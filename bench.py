@@ -82,14 +82,17 @@ def algorithmic_bytes(src_px: int, out_px: int, shadows: bool) -> dict:
     """Compulsory bytes per launch of each stage.
 
     reproject+splat: z 4 + motion 12 read, one 8-byte splat write per source pixel.
-    resolve: splat 8 + colour 4 (+ shadow info 4) read per source pixel, 4 B written per output pixel.
+    resolve: splat 8 + colour 4 (+ shadow info 4) read per source pixel, 4 B written per output pixel,
+    plus the next frame's clear, 8 B written per splat cell (a renderer's resolve clears each view's
+    spare splat buffer, which the next frame splats into; the clear kernel runs only on a renderer's
+    first frame and after a size change).
     clear: 8 B written per splat cell.
     """
     s = 4 if shadows else 0
     return {
         "clear": 8 * src_px,
         "splat": 24 * src_px,
-        "resolve": (12 + s) * src_px + 4 * out_px,
+        "resolve": (12 + s) * src_px + 4 * out_px + 8 * src_px,
     }
 
 
@@ -620,6 +623,9 @@ def main():
         "single_frame_mpix_per_s": round(value_single, 2),
         "p50_frame_ms": round(float(np.percentile(frame_ms, 50)), 4),
         "p99_frame_ms": round(float(np.percentile(frame_ms, 99)), 4),
+        # "clear" is the interval between its stage marks: after a renderer's first
+        # frame the previous frame's resolve has cleared the buffers and no clear
+        # kernel runs (what remains is the cost of the timing events themselves)
         "stage_avg_ms": {k2: round(v2, 5) for k2, v2 in stage_avg_ms.items()},
         # K1: at most one 64-bit atomicMin per source pixel (every in-view one)
         "splat": {
