@@ -351,11 +351,12 @@ bool take_clean(Scratch &s) {
 // null for the renderer's own. Carved planes may hold another call's data, so
 // the flow toward the other keyframe is always recomputed, unless ext_full
 // says that ext's full_ff planes (and full_ok flags) are the renderer's own.
-// The renderer's own views swap in a cleared spare splat buffer (take_clean);
+// The renderer's own views swap in a cleared spare splat buffer (take_clean)
+// unless take_spare is false (a frame already splatted, resolved again);
 // bit (slot * MAX_VIEWS + view) of ext_clean says the same of ext's buffers.
 isti_status plan_frame(isti_renderer *r, const isti_render_info *info, uint8_t *out, size_t out_pitch,
                        FramePlan &P, const Scratch (*ext)[MAX_VIEWS] = nullptr, bool ext_full = false,
-                       uint32_t ext_clean = 0) {
+                       uint32_t ext_clean = 0, bool take_spare = true) {
     const int nv = 1 + r->n_aux;
     Scratch (*scratch)[MAX_VIEWS] = r->scratch;
     if (ext) {
@@ -378,7 +379,7 @@ isti_status plan_frame(isti_renderer *r, const isti_render_info *info, uint8_t *
             const isti_view_desc &d = r->views[sl][v];
             const isti_view_desc &other = r->views[1 - sl][v];
             Scratch &sc = scratch[sl][v];
-            P.clean[P.nj] = ext ? ((ext_clean >> (sl * MAX_VIEWS + v)) & 1u) != 0 : take_clean(sc);
+            P.clean[P.nj] = ext ? ((ext_clean >> (sl * MAX_VIEWS + v)) & 1u) != 0 : take_spare && take_clean(sc);
             SplatJob &J = P.jobs[P.nj++];
             const bool hm = has_motion(d);
             J.z = d.z;
@@ -897,7 +898,9 @@ isti_status isti_render_learned(isti_renderer *r, const isti_render_info *info, 
     if (e != hipSuccess) return dev_fail(r, e, "hipSetDevice");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     FramePlan P;
-    st = plan_frame(r, info, out, out_pitch, P);
+    // the buffers isti_learned_features splatted (a cleared spare stays the
+    // spare: this frame may be resolved again)
+    st = plan_frame(r, info, out, out_pitch, P, nullptr, false, 0, false);
     if (st != ISTI_OK) return st;
     if (!net_shift_for(info->width, info->height, net_w, net_h, P.ra.net_shift))
         return fail(r, ISTI_ERR_UNSUPPORTED_SHAPE, "network grid too small for the render size");
@@ -913,6 +916,8 @@ isti_status isti_render_learned(isti_renderer *r, const isti_render_info *info, 
         if ((e = hipMemsetAsync(r->stat_slots, 0, bytes, s)) != hipSuccess) return dev_fail(r, e, "hipMemsetAsync(stat slots)");
     }
     P.ra.learned_stats = stats ? r->stat_slots : nullptr;
+    // (no clear ahead: in the learned-start kernel it measured no faster, C4
+    // learned path 0.550x -> 0.543x of plain, headline workload 8.05 -> 8.07 Gpx/s)
     if ((e = mark(r, 2, s)) == hipSuccess) e = launch_resolve_learned(P.ra, s);
     if (e == hipSuccess && stats) e = launch_stats_reduce(r->stat_slots, stats, s);
     if (e == hipSuccess) e = mark(r, 3, s);

@@ -470,7 +470,7 @@ __device__ __forceinline__ void clear_ahead(const A &a, uint32_t b) {
 // one wave, or up to 256). Every thread of the block must call it (the LDS
 // staging ends in a barrier); pixels outside the image return early. AHEAD:
 // the kernel clears its share of a.ahead (only the per-render kernel; the
-// batched and learned kernels are built without that code).
+// batched and learned-start kernels are built without that code).
 template <int W, bool MAV, int NAUX, bool LEARNED = false, int TW = TILE_W, int TH = TILE_H, bool AHEAD = false,
           typename A>
 __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveLds<W, NAUX, TW, TH> &lds) {
