@@ -132,7 +132,11 @@ typedef struct isti_renderer isti_renderer;
 
 /* Device scratch of one (slot, view) after the last render, for inspection:
  * the u64 splat buffer and the two flow planes, all device pointers, rows
- * packed: cam_ff (source -> target camera) holds 4 floats per pixel: flow x,
+ * packed. Each view keeps two splat buffers (8 bytes per source pixel each):
+ * isti_render's resolve clears the one the frame did not use (the
+ * reference's clear, renderer.cu:631-638, done a frame early), and the next
+ * frame splats into it without a clear, so `splat` alternates between them
+ * from render to render. cam_ff (source -> target camera) holds 4 floats per pixel: flow x,
  * y, the moved NDC z = RN32(z + flow z) (the reference keeps the flow z and
  * adds the keyframe z where it reads it, renderer.cu:338-341; the sum is
  * stored instead, so a resolve needs no z fetch) and the visible-in-main flag
