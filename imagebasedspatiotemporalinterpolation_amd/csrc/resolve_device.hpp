@@ -637,22 +637,22 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
         chain_start(ch[c][0], ran[c][0], si[c][0].x, si[c][0].y, a.fwd[c]);
         chain_start(ch[c][1], ran[c][1], si[c][1].x, si[c][1].y, a.bwd[c]);
     }
-    uint32_t steps = 0;  // main-camera search steps taken (learned statistics)
+    uint32_t steps = 0;  // main-camera search steps taken (learned statistics only)
     // a learned start gets at most LEARNED_STEPS steps: one still moving by
     // then is judged at that point (and falls back to the window start if
     // rejected), so no wave waits for a wandering learned chain
     const int n_learned = a.max_iter < LEARNED_STEPS ? a.max_iter : LEARNED_STEPS;
     for (int i = 0; i < a.max_iter; i++) {
-        if (LEARNED) {
+        if (LEARNED && i == n_learned) {  // (a stopped chain stays stopped)
 #pragma unroll
             for (int d = 0; d < 2; d++)
-                if (learned[d] && i >= n_learned) ch[0][d].active = false;
+                if (learned[d]) ch[0][d].active = false;
         }
         bool any = false;
 #pragma unroll
         for (int c = 0; c < NC; c++) any = any || ch[c][0].active || ch[c][1].active;
         if (!any) break;
-        if (LEARNED) steps += (uint32_t)ch[0][0].active + (uint32_t)ch[0][1].active;
+        if (LEARNED && a.learned_stats) steps += (uint32_t)ch[0][0].active + (uint32_t)ch[0][1].active;
 #pragma unroll
         for (int c = 0; c < NC; c++) {
             chain_step(ch[c][0], i, (LEARNED && c == 0 && learned[0]) ? n_learned : a.max_iter, pcs, a.fwd[c]);
@@ -686,38 +686,40 @@ __device__ __forceinline__ void resolve_tile(const A &a, uint32_t tile, ResolveL
         // the window start unless its search ends below EPS_1 on a surface
         // not behind the window's nearest splat candidate (the splat's depth
         // test; a fixed point on an occluded surface is valid but hidden)
-        bool redo[2];
+        bool redo[2] = {false, false};
+        if (__ballot(learned[0] || learned[1])) {  // waves without a learned start skip the check
 #pragma unroll
-        for (int d = 0; d < 2; d++) {
-            const bool hidden = window0[d].depth > 0.0f &&
-                                !(depth_to_z(si[0][d].depth) - depth_to_z(window0[d].depth) < EPS_2_FULL_RES);
-            redo[d] = learned[d] && !(err[0][d] < EPS_1_FULL_RES_SQ && !hidden);
-            if (redo[d]) {
-                si[0][d] = window0[d];
-                ran[0][d] = window0[d].depth > 0.0f && (d == 1 || fc.alpha < 1.0f);
+            for (int d = 0; d < 2; d++) {
+                const bool hidden = window0[d].depth > 0.0f &&
+                                    !(depth_to_z(si[0][d].depth) - depth_to_z(window0[d].depth) < EPS_2_FULL_RES);
+                redo[d] = learned[d] && !(err[0][d] < EPS_1_FULL_RES_SQ && !hidden);
+                if (redo[d]) {
+                    si[0][d] = window0[d];
+                    ran[0][d] = window0[d].depth > 0.0f && (d == 1 || fc.alpha < 1.0f);
+                }
+                chain_start(ch[0][d], redo[d] && ran[0][d], si[0][d].x, si[0][d].y, d ? a.bwd[0] : a.fwd[0]);
             }
-            chain_start(ch[0][d], redo[d] && ran[0][d], si[0][d].x, si[0][d].y, d ? a.bwd[0] : a.fwd[0]);
-        }
-        for (int i = 0; i < a.max_iter; i++) {
-            if (!(ch[0][0].active || ch[0][1].active)) break;
-            steps += (uint32_t)ch[0][0].active + (uint32_t)ch[0][1].active;
-            chain_step(ch[0][0], i, a.max_iter, pcs, a.fwd[0]);
-            chain_step(ch[0][1], i, a.max_iter, pcs, a.bwd[0]);
-        }
+            for (int i = 0; i < a.max_iter; i++) {
+                if (!(ch[0][0].active || ch[0][1].active)) break;
+                if (a.learned_stats) steps += (uint32_t)ch[0][0].active + (uint32_t)ch[0][1].active;
+                chain_step(ch[0][0], i, a.max_iter, pcs, a.fwd[0]);
+                chain_step(ch[0][1], i, a.max_iter, pcs, a.bwd[0]);
+            }
 #pragma unroll
-        for (int d = 0; d < 2; d++) {
-            if (!redo[d]) continue;
-            const ResolveView &v = d ? a.bwd[0] : a.fwd[0];
-            err[0][d] = ERR2_NONE;
-            vis[0][d] = false;
-            if (ran[0][d]) {
-                const uint32_t qx = ch[0][d].h0 & 0xFFFFu, qy = ch[0][d].h0 >> 16;
-                const float4 m = at(v.cam_ff, v.w, qx, qy);
-                si[0][d].x = qx;
-                si[0][d].y = qy;
-                si[0][d].depth = z_to_depth(m.z);
-                err[0][d] = reproj_error2(make_uint2(qx, qy), v, cpcs[0], m);
-                vis[0][d] = m.w > 0.0f;
+            for (int d = 0; d < 2; d++) {
+                if (!redo[d]) continue;
+                const ResolveView &v = d ? a.bwd[0] : a.fwd[0];
+                err[0][d] = ERR2_NONE;
+                vis[0][d] = false;
+                if (ran[0][d]) {
+                    const uint32_t qx = ch[0][d].h0 & 0xFFFFu, qy = ch[0][d].h0 >> 16;
+                    const float4 m = at(v.cam_ff, v.w, qx, qy);
+                    si[0][d].x = qx;
+                    si[0][d].y = qy;
+                    si[0][d].depth = z_to_depth(m.z);
+                    err[0][d] = reproj_error2(make_uint2(qx, qy), v, cpcs[0], m);
+                    vis[0][d] = m.w > 0.0f;
+                }
             }
         }
         if (a.learned_stats) {  // one atomic per wave and counter into the block's slot row (ballots see active lanes only)
