@@ -756,8 +756,8 @@ isti_status isti_render_batch(isti_batch *b, isti_renderer *const *rs, const ist
         set_tiles(R.args, BATCH_TILE_SHAPE);  // the batched resolve's tile grid
         // (no clear ahead here: clearing the next batch's buffers in the
         // batched resolve measured slower, 18.3 -> 15.6 Gpx/s on the stereo
-        // frames, whose splat atomics then miss the buffers in the caches a
-        // clear kernel right before the splat leaves them in)
+        // frames, and so did the clear's code alone compiled into that
+        // kernel, unused; the batched resolve is built without it)
         R.vp_cam = P.vp_cam;
         R.vp_main[0] = P.vp_main[0];
         R.vp_main[1] = P.vp_main[1];
