@@ -1252,10 +1252,15 @@ isti_status isti_autotune(isti_renderer *r, const isti_render_info *info, uint8_
             if (x) (void)hipEventDestroy(x);
         return st;
     }
+    // each shape is timed with the clear of the spare splat buffers it does
+    // in a frame (they are cleared over and over here, and left cleared)
     float ms[N_TILE_SHAPES] = {};
     int pick = 0;
+    bool cleared = false;
     for (int k = 0; k < N_TILE_SHAPES && e == hipSuccess; k++) {
         set_tiles(P.ra, k);
+        if ((st = plan_ahead(r, r->scratch, P.ra, k)) != ISTI_OK) break;
+        cleared = cleared || P.ra.ahead.n_bufs > 0;
         e = launch_resolve(P.ra, P.window, P.always_valid, k, s);  // warm-up
         if (e == hipSuccess) e = hipEventRecord(ev[0], s);
         for (int i = 0; i < reps && e == hipSuccess; i++) e = launch_resolve(P.ra, P.window, P.always_valid, k, s);
@@ -1267,7 +1272,9 @@ isti_status isti_autotune(isti_renderer *r, const isti_render_info *info, uint8_
     }
     for (hipEvent_t x : ev)
         if (x) (void)hipEventDestroy(x);
+    if (st != ISTI_OK) return st;
     if (e != hipSuccess) return dev_fail(r, e, "autotune");
+    if (cleared) ahead_cleared(r, r->scratch, P.ra);
     {
         std::lock_guard<std::mutex> lock(g_tuned_mutex);
         g_tuned[TuneKey{P.ra.w, P.ra.h, P.ra.n_aux, P.window}] = pick;
